@@ -1,0 +1,439 @@
+"""MI355X engine module: drop-in for the reference's engine modules.
+
+Same module surface as sieve/gpu_sieve_v9.py / sieve/gpu_sieve_v8_masks.py, which drivers load
+by import name (harness/run_k.py:34, harness/run_k_masks.py:36) or run as a CLI
+(harness/replay_release.py:25): ``Plan``, ``plan_wheel``, ``TUNABLES``, ``GpuSieve(plan)``,
+``free_vram_mib()``, ``load_known(*paths)`` and ``main()``.  ``GpuSieve`` keeps the engine
+object contract those drivers read and write (sieve/gpu_sieve_v9.py:425-568): ``rest_q``,
+``rest_Mmodq``, ``tconst_h`` / ``tconst.set``, ``out_n.fill(0)`` / ``out_n.get()[0]``,
+``out[:n].get()``, ``out_cap``, ``_launch(ostart, ocount)``, ``residues_checked``,
+``launches``, ``r0``, ``R_outer``, ``ochunk``, ``chunk``, ``mask_mode``, ``scan_block(t)`` and
+``search(max_blocks, log)``.  All sieving runs in libes_sieve.so (HIP, gfx950).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import signal
+import sys
+import time
+
+import numpy as np
+
+import es_native as nat
+from es_plan import (MAX_M_BITS, TUNABLES, Plan, allowed_residues, base_digits, is_good,  # noqa: F401
+                     order_filters, partial_count, partial_residues, plan_wheel, primes_upto)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+MASK_MOD_CAP = 4096          # largest prefilter modulus staged in LDS
+MAX_MASKS = 12
+
+
+# --------------------------------------------------------------------------- table building
+
+def pack_bits(q: int, residues) -> np.ndarray:
+    """One bit per residue of [0, q), padded to whole u32 words."""
+    words = np.zeros((q + 31) // 32, dtype=np.uint32)
+    r = np.asarray(residues, dtype=np.int64)
+    if r.size:
+        np.bitwise_or.at(words, r >> 5, (np.uint32(1) << (r & 31).astype(np.uint32)))
+    return words
+
+
+def choose_masks(plan: Plan, n_masks: int, limit: int | None = None):
+    """Word-mask prefilters: for each filter prime p take the deepest low-digit condition
+    mod p^D (T_p < D <= t_p, p^D <= MASK_MOD_CAP) -- a necessary condition of the full test --
+    and keep the n_masks with the smallest conditional pass rate.
+    Returns [(filter_index, D, modulus, conditional pass rate)]."""
+    k = plan.k
+    rest = plan.rest if limit is None else plan.rest[:limit]
+    cands = []
+    for fi, p in enumerate(rest):
+        T = plan.wheelT.get(p, 0)
+        base = partial_count(k, p, T) / p ** T if T else 1.0
+        D = 0
+        for d in range(T + 1, plan.tp[p] + 1):
+            if p ** d <= MASK_MOD_CAP:
+                D = d
+        if D == 0:
+            continue
+        rate = (partial_count(k, p, D) / p ** D) / base
+        cands.append((rate, fi, D, p ** D))
+    cands.sort(key=lambda c: (c[0], c[1]))
+    return [(fi, D, m, rate) for rate, fi, D, m in cands[:max(0, n_masks)]]
+
+
+def slow_order(plan: Plan, masks, limit: int | None = None) -> list[int]:
+    """Filters re-tested exactly per candidate: all of them (first `limit` if given) except
+    those a mask already covers in full, least conditional pass rate (given the masks) first."""
+    k = plan.k
+    rest = plan.rest if limit is None else plan.rest[:limit]
+    by_fi = {fi: D for fi, D, _, _ in masks}
+    keyed = []
+    for fi, p in enumerate(rest):
+        T = plan.wheelT.get(p, 0)
+        D = max(T, by_fi.get(fi, 0))
+        if D >= plan.tp[p]:
+            continue                      # fully decided by the wheel ring or by its mask
+        done = partial_count(k, p, D) / p ** D if D else 1.0
+        q, res = plan.info[p]
+        keyed.append(((len(res) / q) / done, fi))
+    keyed.sort()
+    return [fi for _, fi in keyed]
+
+
+class _Desc:
+    """es_plan_desc plus the numpy arrays it points into (kept alive together)."""
+
+    def __init__(self, plan: Plan, tun: dict):
+        k = plan.k
+        M = plan.M
+        if plan.rings:
+            mods = [q for q, _ in plan.rings]
+            radix = list(plan.radix)
+            terms = [t for ring in plan.ring_terms() for t in ring]
+        else:
+            mods, radix, terms = [1], [1], [0]
+        self.ring_mod = np.asarray(mods, dtype=np.uint32)
+        self.ring_radix = np.asarray(radix, dtype=np.uint32)
+        self.term_lo = np.asarray([t & ((1 << 64) - 1) for t in terms], dtype=np.uint64)
+        self.term_hi = np.asarray([t >> 64 for t in terms], dtype=np.uint64)
+        limit = tun.get("mi_filter_limit")
+        limit = None if limit is None else int(limit)
+        qs, offs, words = [], [], []
+        nw = 0
+        for p in plan.rest:
+            q, res = plan.info[p]
+            qs.append(q)
+            offs.append(nw)
+            w = pack_bits(q, res)
+            words.append(w)
+            nw += w.size
+        self.filt_q = np.asarray(qs or [2], dtype=np.uint32)
+        self.filt_off = np.asarray(offs or [0], dtype=np.uint32)
+        self.filt_bits = np.concatenate(words) if words else np.zeros(1, dtype=np.uint32)
+        n_masks = min(MAX_MASKS, int(tun.get("mi_masks", 8)))
+        self.masks = choose_masks(plan, n_masks, limit)
+        self.slow = slow_order(plan, self.masks, limit)
+        mm, ms, mo, mw = [], [], [], []
+        nw = 0
+        for fi, D, m, _ in self.masks:
+            p = plan.rest[fi]
+            _, res = partial_residues(k, p, D)
+            w = pack_bits(m, res)
+            mm.append(m)
+            ms.append(fi)
+            mo.append(nw)
+            mw.append(w)
+            nw += w.size
+        self.mask_mod = np.asarray(mm or [2], dtype=np.uint32)
+        self.mask_src = np.asarray(ms or [0], dtype=np.uint32)
+        self.mask_off = np.asarray(mo or [0], dtype=np.uint32)
+        self.mask_bits = np.concatenate(mw) if mw else np.zeros(1, dtype=np.uint32)
+        self.slow_arr = np.asarray(self.slow or [0], dtype=np.uint32)
+        d = nat.PlanDesc()
+        d.abi_version = nat.ES_ABI_VERSION
+        d.k = k
+        d.n_rings = len(mods)
+        d.ring_mod = nat.ptr32(self.ring_mod)
+        d.ring_radix = nat.ptr32(self.ring_radix)
+        d.term_lo = nat.ptr64(self.term_lo)
+        d.term_hi = nat.ptr64(self.term_hi)
+        d.M_lo = M & ((1 << 64) - 1)
+        d.M_hi = M >> 64
+        d.n_filters = len(qs)
+        d.filt_q = nat.ptr32(self.filt_q)
+        d.filt_off = nat.ptr32(self.filt_off)
+        d.filt_bits = nat.ptr32(self.filt_bits)
+        d.n_filt_words = int(self.filt_bits.size)
+        d.n_slow = len(self.slow)
+        d.slow_order = nat.ptr32(self.slow_arr)
+        d.n_masks = len(mm)
+        d.mask_mod = nat.ptr32(self.mask_mod)
+        d.mask_src = nat.ptr32(self.mask_src)
+        d.mask_off = nat.ptr32(self.mask_off)
+        d.mask_bits = nat.ptr32(self.mask_bits)
+        d.n_mask_words = int(self.mask_bits.size)
+        d.strip_target = int(tun.get("mi_strip_target", 512))
+        d.threads = int(tun.get("threads", 256))
+        d.out_cap = int(tun.get("max_survivors", 1 << 20))
+        d.lds_budget = int(tun.get("mi_lds_budget", 65536))
+        self.desc = d
+
+
+# --------------------------------------------------------------------------- device proxies
+
+class _Counter:
+    """out_n: the device survivor counter (fill(0) resets it, get() reads it)."""
+
+    def __init__(self, eng: "GpuSieve"):
+        self._e = eng
+
+    def fill(self, v: int) -> None:
+        assert v == 0, "the survivor counter can only be reset to 0"
+        nat.check(nat.lib().es_engine_reset(self._e._h))
+
+    def get(self) -> np.ndarray:
+        n = C.c_uint64(0)
+        nat.check(nat.lib().es_engine_count(self._e._h, C.byref(n)))
+        return np.asarray([n.value], dtype=np.uint64)
+
+
+class _OutSlice:
+    def __init__(self, eng: "GpuSieve", lo: int, hi: int):
+        self._e, self._lo, self._hi = eng, lo, hi
+
+    def get(self) -> np.ndarray:
+        n = max(0, self._hi - self._lo)
+        buf = np.zeros(n, dtype=np.uint64)
+        if n:
+            nat.check(nat.lib().es_engine_read(self._e._h, self._lo, n, nat.ptr64(buf)))
+        return buf
+
+
+class _Out:
+    """out: the device survivor buffer; out[a:b].get() copies IDs back."""
+
+    def __init__(self, eng: "GpuSieve"):
+        self._e = eng
+
+    def __getitem__(self, sl) -> _OutSlice:
+        assert isinstance(sl, slice) and sl.step in (None, 1)
+        lo = 0 if sl.start is None else int(sl.start)
+        hi = self._e.out_cap if sl.stop is None else min(int(sl.stop), self._e.out_cap)
+        return _OutSlice(self._e, lo, hi)
+
+
+class _TConst:
+    """tconst: per-block filter constants; set(arr) uploads arr (u32, one per filter)."""
+
+    def __init__(self, eng: "GpuSieve"):
+        self._e = eng
+
+    def set(self, arr) -> None:
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.uint32)[:len(self._e.rest_q)])
+        nat.check(nat.lib().es_engine_set_tconst(self._e._h, nat.ptr32(a), len(self._e.rest_q)))
+
+
+# --------------------------------------------------------------------------- engine
+
+class GpuSieve:
+    """One wheel plan on the current GPU (sieve/gpu_sieve_v9.py:425-568)."""
+
+    def __init__(self, plan: Plan):
+        self.plan = plan
+        P = plan
+        self._h = None
+        L = nat.lib()
+        self._d = _Desc(P, P.tun)
+        h = C.c_void_p()
+        nat.check(L.es_engine_create(C.byref(self._d.desc), C.byref(h)))
+        self._h = h
+        info = nat.EngineInfo()
+        nat.check(L.es_engine_info_get(h, C.byref(info)))
+        self.info = info
+        self.w = len(P.wheel)
+        self.rest_q = [P.info[p][0] for p in P.rest]
+        self.rest_Mmodq = [P.M % q for q in self.rest_q]
+        self.nrest = len(P.rest)
+        self.tconst_h = np.zeros(max(1, self.nrest), dtype=np.uint32)
+        self.tconst = _TConst(self)
+        self.out_cap = int(P.tun["max_survivors"])
+        self.out = _Out(self)
+        self.out_n = _Counter(self)
+        self.r0 = P.radix[0] if P.radix else 1
+        self.n_fast = info.n_masks_used
+        self.threads = int(P.tun["threads"])
+        self.chunk = 1 << int(P.tun.get("mi_launch_log2", 36))
+        self.R_outer = P.R // self.r0
+        assert self.R_outer == info.R_outer and self.R_outer * self.r0 == P.R
+        self.ochunk = max(1, self.chunk // self.r0)
+        self.strip_len = info.strip_len
+        self.mask_mode = f"mi355x-lds-mask{info.n_masks_used}"
+        self.masks = self._d.masks
+        self.launches = 0
+        self.residues_checked = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                nat.lib().es_engine_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- reference contract ------------------------------------------------------------
+    def _launch(self, ostart: int, ocount: int) -> None:
+        """ostart/ocount are OUTER indices; each covers r0 positions.  Asynchronous."""
+        nat.check(nat.lib().es_engine_scan(self._h, int(ostart), int(ocount)))
+        self.launches += 1
+
+    def set_block(self, t: int) -> None:
+        for j, q in enumerate(self.rest_q):
+            self.tconst_h[j] = ((t % q) * self.rest_Mmodq[j]) % q
+        self.tconst.set(self.tconst_h)
+
+    def scan_block(self, t: int) -> list[int]:
+        """All positions of block t; one sync per block.  Survivor IDs are block-absolute."""
+        self.set_block(t)
+        self.out_n.fill(0)
+        ostart = 0
+        while ostart < self.R_outer:
+            ocount = min(self.ochunk, self.R_outer - ostart)
+            self._launch(ostart, ocount)
+            self.residues_checked += ocount * self.r0
+            ostart += ocount
+        n = int(self.out_n.get()[0])
+        if n > self.out_cap:
+            return self._scan_block_chunked(t)
+        return [int(x) for x in self.out[:n].get()] if n else []
+
+    def _scan_block_chunked(self, t: int) -> list[int]:
+        """Per-launch drain, used only when a block overflows the buffer."""
+        survivors = []
+        ostart = 0
+        while ostart < self.R_outer:
+            ocount = min(self.ochunk, self.R_outer - ostart)
+            self.out_n.fill(0)
+            self._launch(ostart, ocount)
+            n = int(self.out_n.get()[0])
+            if n > self.out_cap:
+                raise RuntimeError(f"survivor buffer overflow ({n} > {self.out_cap}); "
+                                   "raise max_survivors")
+            if n:
+                survivors.extend(int(x) for x in self.out[:n].get())
+            ostart += ocount
+        return survivors
+
+    def search(self, max_blocks: int | None = None, log=None) -> dict:
+        P = self.plan
+        t0 = time.time()
+        t = 0
+        while max_blocks is None or t < max_blocks:
+            surv = self.scan_block(t)
+            cands = sorted(P.idx_to_n(i, t) for i in surv)
+            good = [n for n in cands if is_good(n, P.k)]
+            bad = [n for n in cands if n > P.k + 1 and n not in good]
+            if bad:
+                return {"k": P.k, "status": "DISAGREEMENT", "kernel_survivors": cands[:20],
+                        "ref_rejected": bad[:20], "block": t, "wall": time.time() - t0}
+            if log and (good or time.time() - getattr(self, "_last_log", 0) > 10):
+                self._last_log = time.time()
+                log(f"k={P.k} block t={t} survivors={len(surv)} good={len(good)} "
+                    f"checked={self.residues_checked:.3e} wall={time.time()-t0:.1f}s")
+            if good:
+                return {"k": P.k, "status": "FOUND", "g": good[0], "block": t,
+                        "survivors_in_block": len(surv), "residues_checked": self.residues_checked,
+                        "launches": self.launches, "wall": time.time() - t0}
+            t += 1
+        return {"k": P.k, "status": "EXHAUSTED_BLOCKS", "blocks": t,
+                "residues_checked": self.residues_checked, "wall": time.time() - t0}
+
+    # -- windows (positions) -------------------------------------------------------------
+    def scan_window(self, t: int, idx_lo: int, idx_hi: int, collect: bool = True):
+        """Survivor IDs with idx in [idx_lo, idx_hi) of block t, and the least n > k+1 among
+        them (or None).  Returns (ids, min_n)."""
+        self.set_block(t)
+        self.out_n.fill(0)
+        nat.check(nat.lib().es_engine_scan_window(self._h, int(idx_lo), int(idx_hi)))
+        self.residues_checked += int(idx_hi) - int(idx_lo)
+        if not collect:
+            return None
+        return self.collect(t)
+
+    def collect(self, t: int):
+        n = C.c_uint64(0)
+        nat.check(nat.lib().es_engine_count(self._h, C.byref(n)))
+        cnt = n.value
+        if cnt > self.out_cap:
+            raise RuntimeError(f"survivor buffer overflow ({cnt} > {self.out_cap}); "
+                               "raise max_survivors")
+        ids = np.zeros(max(cnt, 1), dtype=np.uint64)
+        ms = np.zeros(2, dtype=np.uint64)
+        has = C.c_int(0)
+        nat.check(nat.lib().es_engine_collect(self._h, nat.ptr64(ids), cnt, C.byref(n),
+                                              nat.ptr64(ms), C.byref(has)))
+        min_n = None
+        if has.value:
+            min_n = t * self.plan.M + (int(ms[1]) << 64 | int(ms[0]))
+        return [int(x) for x in ids[:cnt]], min_n
+
+    def kernel_time(self):
+        ms = C.c_double(0)
+        la, pos = C.c_uint64(0), C.c_uint64(0)
+        nat.check(nat.lib().es_engine_kernel_time(self._h, C.byref(ms), C.byref(la), C.byref(pos)))
+        return ms.value, la.value, pos.value
+
+    def sync(self):
+        nat.check(nat.lib().es_engine_sync(self._h))
+
+
+# --------------------------------------------------------------------------- module helpers
+
+def free_vram_mib() -> int | None:
+    """Free memory of the current GPU in MiB (sieve/gpu_sieve_v9.py:416-422), None if unknown."""
+    try:
+        return int(nat.device_query()["free_mib"])
+    except Exception:
+        return None
+
+
+def load_known(*paths: str) -> dict[int, tuple[int, str]]:
+    """k -> (g, source) from whitespace 'k g' files; later files do not override earlier."""
+    known: dict[int, tuple[int, str]] = {}
+    for path in paths:
+        if not path or not os.path.exists(path):
+            continue
+        for line in open(path):
+            parts = line.split()
+            if len(parts) >= 2 and parts[0].isdigit() and parts[1].isdigit():
+                known.setdefault(int(parts[0]), (int(parts[1]), os.path.basename(path)))
+    return known
+
+
+def main(argv=None):
+    """CLI with the argv and one-JSON-line-per-k output of sieve/gpu_sieve_v9.py:584-618."""
+    ap = argparse.ArgumentParser(description="MI355X CRT-wheel search for g(k), partial-power wheel")
+    ap.add_argument("k", type=int, nargs="+")
+    ap.add_argument("--plan-only", action="store_true")
+    ap.add_argument("--tune", type=json.loads, default=None, help="JSON dict overriding TUNABLES")
+    ap.add_argument("--wheel", type=json.loads, default=None,
+                    help="explicit wheel: JSON list of [p,T] (or bare p = full power)")
+    ap.add_argument("--max-blocks", type=int, default=None)
+    ap.add_argument("--timeout", type=float, default=None, help="seconds; process exits 124 on expiry")
+    ap.add_argument("--min-free-mib", type=int, default=2048)
+    ap.add_argument("--bfile", default=os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt"))
+    ap.add_argument("--known", default=None, help="extra 'k g' file of known terms")
+    a = ap.parse_args(argv)
+    if a.timeout:
+        signal.signal(signal.SIGALRM, lambda *_: (print(json.dumps({"status": "TIMEOUT"}), flush=True),
+                                                  os._exit(124)))
+        signal.setitimer(signal.ITIMER_REAL, a.timeout)
+    known = load_known(a.bfile, a.known)
+    for k in a.k:
+        plan = Plan(k, a.tune, a.wheel)
+        if a.plan_only:
+            print(json.dumps(plan.summary()))
+            continue
+        free = free_vram_mib()
+        if free is not None and free < a.min_free_mib:
+            print(json.dumps({"k": k, "status": "SKIPPED_LOW_VRAM", "free_mib": free}))
+            continue
+        sieve = GpuSieve(plan)
+        res = sieve.search(a.max_blocks, log=lambda s: print(s, file=sys.stderr, flush=True))
+        if res.get("status") == "FOUND" and k in known:
+            g_known, src = known[k]
+            res["matches_bfile"] = (res["g"] == g_known)
+            res["bfile"] = g_known
+            res["known_source"] = src
+        res["plan"] = plan.summary()
+        print(json.dumps(res, default=str), flush=True)
+
+
+if __name__ == "__main__":
+    main()

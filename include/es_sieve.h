@@ -1,0 +1,131 @@
+/* es_sieve.h -- C ABI of the MI355X (gfx950) CRT-wheel sieve engine for the Erdos-Selfridge
+ * function g(k).
+ *
+ * One engine = one wheel plan on one GPU, one HIP stream.  The caller describes the plan with
+ * plain arrays (ring residues and CRT terms, filter admissibility bitmaps, word-mask
+ * prefilters); the library builds every derived device table itself.  A scan enumerates the
+ * residue positions idx of block t (n in [tM, (t+1)M)), keeps every idx whose n passes all
+ * filters, and appends it to the engine's survivor buffer.
+ *
+ * Reference interface replaced: the duck-typed engine object of sieve/gpu_sieve_v9.py:425-568
+ * (GpuSieve: constructor, tconst.set, out_n.fill(0), _launch(ostart, ocount), out_n.get(),
+ * out[:n].get()) and its word-mask variant sieve/es_word_masks.py:95-162.  Survivor IDs are
+ * block-relative mixed-radix indices in the ORIGINAL ring order (ring 0 fastest), exactly as the
+ * reference emits them (sieve/gpu_sieve_v9.py:325, sieve/es_word_masks.py:43-44,88).
+ *
+ * Rules: every call returns 0 on success or a negative es_status; no exception crosses the
+ * ABI; es_last_error() gives the message of the last failure on the calling thread.  The
+ * caller owns every array it passes (they are copied during es_engine_create) and every
+ * output buffer.  An engine must be used from one host thread at a time.  Scans are
+ * asynchronous; es_engine_count/read/collect/sync synchronise the engine's stream.
+ */
+#ifndef ES_SIEVE_H
+#define ES_SIEVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ES_ABI_VERSION 1u
+
+typedef enum es_status {
+  ES_OK = 0,
+  ES_ERR_INVALID = -1,    /* bad argument or inconsistent plan (reference: AssertionError) */
+  ES_ERR_DEVICE = -2,     /* HIP runtime failure, or no usable GPU */
+  ES_ERR_OVERFLOW = -3,   /* survivor buffer overflow (reference: RuntimeError) */
+  ES_ERR_LIMIT = -4       /* plan exceeds a documented limit of this engine */
+} es_status;
+
+typedef struct es_engine es_engine;
+
+/* Plan descriptor.  Rings are in the reference's ring order (Plan.wheel after the
+ * (-radix, p) sort, sieve/gpu_sieve_v9.py:366); ring 0 varies fastest in idx.  An empty
+ * wheel is passed as one ring with modulus 1, radix 1, residue 0, term 0. */
+typedef struct es_plan_desc {
+  uint32_t abi_version;        /* ES_ABI_VERSION */
+  uint32_t k;
+  uint32_t n_rings;            /* w >= 1 */
+  const uint32_t* ring_mod;    /* [w] q_j = p^T (< 2^32) */
+  const uint32_t* ring_radix;  /* [w] number of admissible residues of ring j */
+  const uint64_t* term_lo;     /* [sum radix] CRT term (a * basis_j) mod M, ring-major, */
+  const uint64_t* term_hi;     /*             residues of a ring in ascending order     */
+  uint64_t M_lo, M_hi;         /* wheel modulus M < 2^126 */
+  uint32_t n_filters;          /* filter primes in kernel order (Plan.rest) */
+  const uint32_t* filt_q;      /* [n_filters] q = p^t_p  (< 2^24) */
+  const uint32_t* filt_off;    /* [n_filters] word offset of each bitmap in filt_bits */
+  const uint32_t* filt_bits;   /* bit r of table j set iff residue r mod q_j is admissible */
+  uint32_t n_filt_words;       /* total words in filt_bits */
+  uint32_t n_slow;             /* filters evaluated exactly per candidate, in this order */
+  const uint32_t* slow_order;  /* [n_slow] indices into the filter list */
+  uint32_t n_masks;            /* word-mask prefilters: necessary conditions only */
+  const uint32_t* mask_mod;    /* [n_masks] modulus m (a divisor of its filter's q) */
+  const uint32_t* mask_src;    /* [n_masks] index of the filter the mask derives from */
+  const uint32_t* mask_off;    /* [n_masks] word offset of each bitmap in mask_bits */
+  const uint32_t* mask_bits;   /* bit r set iff residue r mod m passes the mask */
+  uint32_t n_mask_words;
+  uint32_t strip_target;       /* target outer indices per uniform strip (0 = default) */
+  uint32_t threads;            /* workgroup size: 64, 128, 256, 512 or 1024 (0 = 256) */
+  uint64_t out_cap;            /* survivor buffer capacity in u64 IDs (0 = 2^20) */
+  uint32_t lds_budget;         /* max bytes of LDS tables per workgroup (0 = 64 KiB) */
+  uint32_t reserved;
+} es_plan_desc;
+
+typedef struct es_engine_info {
+  uint64_t R_lo, R_hi;         /* positions per block */
+  uint64_t R_outer;            /* outer indices per block (R / r0) */
+  uint64_t strips;             /* strips per block */
+  uint32_t r0, q0;             /* ring-0 radix and modulus */
+  uint32_t strip_len;          /* outer indices per strip */
+  uint32_t n_masks_used;       /* masks staged in LDS (may be < n_masks if LDS is short) */
+  uint32_t lds_bytes;          /* LDS tables per workgroup */
+  uint32_t words;              /* u32 words per ring-0 mask row */
+  uint32_t threads;
+  uint32_t device;
+} es_engine_info;
+
+/* Last error message of the calling thread ("" if none).  Never NULL. */
+const char* es_last_error(void);
+
+/* Number of visible GPUs and free/total memory of the current one, in MiB.
+ * Counterpart of free_vram_mib (sieve/gpu_sieve_v9.py:416-422). */
+int es_device_query(int* n_devices, uint64_t* free_mib, uint64_t* total_mib, char* name, uint32_t name_len);
+
+/* Build an engine for a plan on the current HIP device (GpuSieve.__init__,
+ * sieve/gpu_sieve_v9.py:426-498).  The survivor counter starts at zero and t = 0. */
+int es_engine_create(const es_plan_desc* desc, es_engine** out);
+int es_engine_destroy(es_engine* e);
+int es_engine_info_get(const es_engine* e, es_engine_info* info);
+
+/* Per-block constants tconst[j] = (t*M) mod q_j for every filter j, as the reference drivers
+ * compute and upload them (sieve/gpu_sieve_v9.py:512-514, harness/run_k_masks.py:131-134). */
+int es_engine_set_tconst(es_engine* e, const uint32_t* tconst, uint32_t n);
+/* Same, computed by the library from the block index t. */
+int es_engine_set_block(es_engine* e, uint64_t t);
+
+/* Zero the survivor counter (out_n.fill(0)). */
+int es_engine_reset(es_engine* e);
+/* Scan outer indices [ostart, ostart+ocount); each covers r0 positions (GpuSieve._launch,
+ * sieve/gpu_sieve_v9.py:500-506).  Asynchronous. */
+int es_engine_scan(es_engine* e, uint64_t ostart, uint64_t ocount);
+/* Scan positions [idx_lo, idx_hi) of the current block.  Asynchronous. */
+int es_engine_scan_window(es_engine* e, uint64_t idx_lo, uint64_t idx_hi);
+/* Synchronise; *n = survivors appended since the last reset (may exceed out_cap). */
+int es_engine_count(es_engine* e, uint64_t* n);
+/* Synchronise; copy survivor IDs [offset, offset+count) (count <= out_cap - offset). */
+int es_engine_read(es_engine* e, uint64_t offset, uint64_t count, uint64_t* ids);
+/* Synchronise; copy up to cap survivor IDs into ids, set *n to the count, and set
+ * min_s = {lo, hi} to the least residue s (n = tM + s) over survivors with n > k+1
+ * (*has_min = 0 if there is none).  Returns ES_ERR_OVERFLOW if the buffer overflowed. */
+int es_engine_collect(es_engine* e, uint64_t* ids, uint64_t cap, uint64_t* n,
+                      uint64_t min_s[2], int* has_min);
+int es_engine_sync(es_engine* e);
+/* Device time of the sieve kernels launched since the last call (HIP events on the engine's
+ * stream), their launch count, and the positions they covered. */
+int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t* positions);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ES_SIEVE_H */

@@ -1,0 +1,232 @@
+/* es_oracle.c -- CPU ORACLE for the CRT-wheel sieve.  TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, and
+ * only as the checker or as the timed CPU baseline.  The engine never calls it.
+ *
+ * It restates the reference's CPU Kummer path on the wheel enumeration:
+ *   - ref/erdos_ref.py:31-48      digits / dominates (Kummer digit domination)
+ *   - ref/erdos_ref.py:69-90      allowed residues per ring and the CRT basis M_i*(M_i^-1 mod q)
+ *   - sieve/gpu_sieve_v9.py:127-138  partial_residues (T low digits), ascending order
+ *   - sieve/gpu_sieve_v9.py:385-390  idx_to_n: mixed radix, ring 0 fastest, n = t*M + (sum mod M)
+ *   - tests/test_enumeration.py:37-43 the per-position loop: idx_to_n, then dominates for every
+ *                                   filter prime of the plan.
+ * Every table is derived here from (k, wheel items, filter primes) with plain integer loops; no
+ * table built by the engine's host code is reused.  n mod p^t_p replaces n in dominates(): the
+ * digit loop stops after the t_p digits of k, so only n mod p^t_p is ever read.
+ */
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+#define ESO_MAX_RINGS 64
+#define ESO_MAX_FILT 512
+
+typedef struct {
+  uint32_t k, w, nf;
+  uint32_t ring_q[ESO_MAX_RINGS], ring_r[ESO_MAX_RINGS];
+  u128* terms[ESO_MAX_RINGS];
+  u128 M;
+  uint32_t fp[ESO_MAX_FILT], fq[ESO_MAX_FILT], ftc[ESO_MAX_FILT];
+} eso_plan;
+
+static uint64_t ipow(uint64_t p, uint32_t e) {
+  uint64_t r = 1;
+  while (e--) r *= p;
+  return r;
+}
+
+static uint32_t ndigits(uint32_t k, uint32_t p) {
+  uint32_t n = 0;
+  while (k) { k /= p; ++n; }
+  return n;
+}
+
+/* dominates(n, k, p) with n given as n mod p^t_p  (ref/erdos_ref.py:40-48) */
+static int dominates_r(uint64_t r, uint32_t k, uint32_t p) {
+  while (k) {
+    if (k % p > r % p) return 0;
+    k /= p;
+    r /= p;
+  }
+  return 1;
+}
+
+/* T low base-p digits of r dominate those of k */
+static int low_digits_ok(uint64_t r, uint32_t k, uint32_t p, uint32_t T) {
+  for (uint32_t i = 0; i < T; ++i) {
+    if (k % p > r % p) return 0;
+    k /= p;
+    r /= p;
+  }
+  return 1;
+}
+
+static int64_t inv_mod(int64_t a, int64_t m) {  /* extended Euclid, gcd(a, m) = 1 */
+  int64_t g = m, x = 0, x1 = 1, a1 = a % m;
+  if (a1 < 0) a1 += m;
+  while (a1) {
+    int64_t q = g / a1, t;
+    t = g - q * a1; g = a1; a1 = t;
+    t = x - q * x1; x = x1; x1 = t;
+  }
+  if (g != 1) return -1;
+  x %= m;
+  if (x < 0) x += m;
+  return x;
+}
+
+static void eso_free(eso_plan* P) {
+  for (uint32_t j = 0; j < P->w; ++j) free(P->terms[j]);
+  free(P);
+}
+
+/* Build the plan: rings in the given order (ring 0 fastest in idx). */
+static eso_plan* eso_build(uint32_t k, uint32_t w, const uint32_t* rp, const uint32_t* rT,
+                           uint32_t nf, const uint32_t* fp, uint64_t t) {
+  if (w > ESO_MAX_RINGS || nf > ESO_MAX_FILT) return NULL;
+  eso_plan* P = (eso_plan*)calloc(1, sizeof(eso_plan));
+  P->k = k; P->w = w; P->nf = nf;
+  P->M = 1;
+  for (uint32_t j = 0; j < w; ++j) {
+    P->ring_q[j] = (uint32_t)ipow(rp[j], rT[j]);
+    P->M *= P->ring_q[j];
+  }
+  for (uint32_t j = 0; j < w; ++j) {
+    const uint32_t p = rp[j], q = P->ring_q[j];
+    const u128 Mi = P->M / q;
+    const int64_t inv = inv_mod((int64_t)(Mi % q), q);
+    if (inv < 0) { P->w = j; eso_free(P); return NULL; }
+    uint32_t cnt = 0;
+    for (uint32_t r = 0; r < q; ++r) cnt += (uint32_t)low_digits_ok(r, k, p, rT[j]);
+    P->ring_r[j] = cnt;
+    P->terms[j] = (u128*)malloc(sizeof(u128) * (cnt ? cnt : 1));
+    uint32_t i = 0;
+    for (uint32_t r = 0; r < q; ++r)
+      if (low_digits_ok(r, k, p, rT[j]))  /* ascending residues */
+        P->terms[j][i++] = Mi * (u128)(((uint64_t)r * (uint64_t)inv) % q);
+  }
+  for (uint32_t f = 0; f < nf; ++f) {
+    P->fp[f] = fp[f];
+    P->fq[f] = (uint32_t)ipow(fp[f], ndigits(k, fp[f]));
+    P->ftc[f] = (uint32_t)(((u128)(t % P->fq[f]) * (uint64_t)(P->M % P->fq[f])) % P->fq[f]);
+  }
+  return P;
+}
+
+typedef struct {
+  const eso_plan* P;
+  uint64_t lo, hi;
+  uint64_t* out;
+  uint64_t cap, n;
+} eso_job;
+
+static void* eso_worker(void* arg) {
+  eso_job* J = (eso_job*)arg;
+  const eso_plan* P = J->P;
+  for (uint64_t idx = J->lo; idx < J->hi; ++idx) {
+    /* idx_to_n: mixed radix, ring 0 fastest */
+    uint64_t rem = idx;
+    u128 s = 0;
+    for (uint32_t j = 0; j < P->w; ++j) {
+      const uint64_t d = rem % P->ring_r[j];
+      rem /= P->ring_r[j];
+      s += P->terms[j][d];
+      if (s >= P->M) s -= P->M;
+    }
+    int ok = 1;
+    for (uint32_t f = 0; f < P->nf && ok; ++f) {
+      const uint32_t q = P->fq[f];
+      const uint64_t r = ((uint64_t)(s % q) + P->ftc[f]) % q;
+      ok = dominates_r(r, P->k, P->fp[f]);
+    }
+    if (ok) {
+      if (J->n < J->cap) J->out[J->n] = idx;
+      J->n++;
+    }
+  }
+  return NULL;
+}
+
+/* Survivors of positions [idx_lo, idx_hi) of block t, ascending.  Returns the count (which may
+ * exceed cap; only cap IDs are written), or -1 on an invalid plan. */
+int64_t eso_scan(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
+                 uint32_t nf, const uint32_t* filt_p, uint64_t t, uint64_t idx_lo,
+                 uint64_t idx_hi, uint32_t nthreads, uint64_t* out, uint64_t cap) {
+  eso_plan* P = eso_build(k, w, ring_p, ring_T, nf, filt_p, t);
+  if (!P) return -1;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  const uint64_t span = idx_hi > idx_lo ? idx_hi - idx_lo : 0;
+  if (span < (uint64_t)nthreads * 1024) nthreads = 1;
+  eso_job* jobs = (eso_job*)calloc(nthreads, sizeof(eso_job));
+  pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+  for (uint32_t i = 0; i < nthreads; ++i) {
+    jobs[i].P = P;
+    jobs[i].lo = idx_lo + span * i / nthreads;
+    jobs[i].hi = idx_lo + span * (i + 1) / nthreads;
+    jobs[i].cap = cap;
+    jobs[i].out = (uint64_t*)malloc(sizeof(uint64_t) * (cap ? cap : 1));
+    pthread_create(&th[i], NULL, eso_worker, &jobs[i]);
+  }
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nthreads; ++i) {
+    pthread_join(th[i], NULL);
+    const uint64_t take = jobs[i].n < cap ? jobs[i].n : cap;
+    for (uint64_t j = 0; j < take && total + j < cap; ++j) out[total + j] = jobs[i].out[j];
+    total += jobs[i].n;
+    free(jobs[i].out);
+  }
+  free(jobs);
+  free(th);
+  eso_free(P);
+  return (int64_t)total;
+}
+
+/* Plan geometry for cross-checks: M (lo, hi), R, and the CRT term of (ring j, slot i). */
+int eso_geometry(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
+                 uint64_t M_out[2], uint64_t* R_out, uint32_t* radix_out) {
+  eso_plan* P = eso_build(k, w, ring_p, ring_T, 0, NULL, 0);
+  if (!P) return -1;
+  M_out[0] = (uint64_t)P->M;
+  M_out[1] = (uint64_t)(P->M >> 64);
+  u128 R = 1;
+  for (uint32_t j = 0; j < w; ++j) {
+    radix_out[j] = P->ring_r[j];
+    R *= P->ring_r[j];
+  }
+  *R_out = (uint64_t)R;
+  eso_free(P);
+  return 0;
+}
+
+/* n = t*M + s of position idx (for inverse-map and minimum checks). */
+int eso_idx_to_s(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
+                 uint64_t idx, uint64_t s_out[2]) {
+  eso_plan* P = eso_build(k, w, ring_p, ring_T, 0, NULL, 0);
+  if (!P) return -1;
+  u128 s = 0;
+  for (uint32_t j = 0; j < w; ++j) {
+    const uint64_t d = idx % P->ring_r[j];
+    idx /= P->ring_r[j];
+    s += P->terms[j][d];
+    if (s >= P->M) s -= P->M;
+  }
+  s_out[0] = (uint64_t)s;
+  s_out[1] = (uint64_t)(s >> 64);
+  eso_free(P);
+  return 0;
+}
+
+/* Kummer test of one n (< 2^128) for one prime: dominates(n, k, p). */
+int eso_dominates(uint64_t n_lo, uint64_t n_hi, uint32_t k, uint32_t p) {
+  u128 n = ((u128)n_hi << 64) | n_lo;
+  while (k) {
+    if (k % p > (uint32_t)(n % p)) return 0;
+    k /= p;
+    n /= p;
+  }
+  return 1;
+}

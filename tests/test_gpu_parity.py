@@ -1,0 +1,155 @@
+"""GPU parity: survivors of the HIP engine (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact survivor ID sets (compared sorted), exact minima, b-file values.  Every case
+prints its name, plan and window before touching the GPU."""
+from __future__ import annotations
+
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import es_oracle as O
+import es_plan as EP
+from conftest import GOLDEN, bfile
+
+pytestmark = pytest.mark.gpu
+
+C1_WHEEL = [[2, 5], [3, 4], [5, 3], [7, 2], [11, 2], [13, 2]]
+K399_PRIMARY = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+                [41, 1], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]]
+K399_CONFIRM = [[223, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+                [5, 4], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1]]
+
+
+def engine(k, wheel=None, **tune):
+    import gpu_sieve_mi355x as G
+    plan = EP.Plan(k, dict(tune), wheel)
+    return G.GpuSieve(plan), plan
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def window_check(k, wheel, t, lo, hi, limit=None, **tune):
+    if limit is not None:
+        tune["mi_filter_limit"] = limit
+    tune.setdefault("max_survivors", 1 << 22)
+    eng, plan = engine(k, wheel, **tune)
+    filt = plan.rest if limit is None else plan.rest[:limit]
+    say(f"case k={k} t={t} window=[{lo},{hi}) filters={len(filt)} masks={eng.info.n_masks_used} "
+        f"strip={eng.strip_len} r0={eng.r0}")
+    ids, mn = eng.scan_window(t, lo, hi)
+    got = np.sort(np.asarray(ids, dtype=np.uint64))
+    n, want = O.c_scan(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
+    assert n == len(want)
+    assert len(got) == n, f"count {len(got)} != oracle {n}"
+    assert np.array_equal(got, want)
+    # minimum over n > k+1 agrees with the oracle's idx -> n map
+    ns = [O.c_idx_to_n(k, plan.wheel, int(i), t) for i in want[:2000]] if n <= 2000 else None
+    if ns is not None:
+        valid = [x for x in ns if x > k + 1]
+        assert mn == (min(valid) if valid else None)
+    return n
+
+
+def test_config1_window_dense():
+    assert window_check(30, C1_WHEEL, 0, 0, 1 << 24) > 100000
+
+
+def test_config1_unaligned_windows():
+    rng = random.Random(1)
+    for _ in range(4):
+        lo = rng.randrange(0, 230947200 - (1 << 20))
+        hi = lo + rng.randrange(1, 1 << 20)
+        window_check(30, C1_WHEEL, rng.randrange(0, 50), lo, hi)
+
+
+def test_config1_full_block_min():
+    eng, plan = engine(30, C1_WHEEL, max_survivors=1 << 24)
+    say("case k=30 full block 0")
+    ids, mn = eng.scan_window(0, 0, plan.R)
+    n, want = O.c_scan(30, plan.wheel, plan.rest, 0, 0, plan.R, threads=16, cap=1 << 24)
+    assert len(ids) == n == 12654618
+    assert np.array_equal(np.sort(np.asarray(ids, dtype=np.uint64)), want)
+    assert mn == 58782
+
+
+@pytest.mark.parametrize("k", [100, 200, 399])
+def test_filter_prefix_windows(k):
+    """Only the first few filters: dense survivors at large k, random block and offset."""
+    rng = random.Random(k)
+    wheel = K399_PRIMARY if k == 399 else None
+    plan = EP.Plan(k, None, wheel)
+    for limit in (3, 5):
+        t = rng.randrange(0, 1 << 20)
+        lo = rng.randrange(0, plan.R - (1 << 22))
+        window_check(k, wheel, t, lo, lo + (1 << 22), limit=limit)
+
+
+def test_k399_confirmation_prefix():
+    plan = EP.Plan(399, None, K399_CONFIRM)
+    rng = random.Random(7)
+    lo = rng.randrange(0, plan.R - (1 << 22))
+    window_check(399, K399_CONFIRM, 3, lo, lo + (1 << 22), limit=4)
+
+
+def test_bfile_gate_small_k():
+    import gpu_sieve_mi355x as G
+    b = bfile()
+    for k in range(1, 61):
+        plan = EP.Plan(k, {"max_survivors": 1 << 24})
+        say(f"case gate k={k} M={plan.M} R={plan.R}")
+        res = G.GpuSieve(plan).search(max_blocks=400)
+        assert res["status"] == "FOUND", res
+        assert res["g"] == b[k], (k, res["g"], b[k])
+
+
+def _frontier():
+    return json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
+
+
+@pytest.mark.parametrize("k,role,t_idx", [(380, "primary", 0), (399, "primary", 0),
+                                          (399, "confirmation", 0), (398, "confirmation", 0),
+                                          (400, "primary", 1)])
+def test_recorded_survivor_windows(k, role, t_idx):
+    """Windows of 2^24 positions around every recorded survivor of a recorded block must
+    return exactly the recorded survivors inside the window (full filter set)."""
+    term = next(x for x in _frontier() if x["k"] == k)
+    run = term[role]
+    audit = run["block_audit"][t_idx]
+    t = audit["t"]
+    eng, plan = engine(k, run["wheel"], max_survivors=1 << 16)
+    assert str(plan.M) == run["N"] and plan.R == run["residues_per_block"]
+    for sv in audit["survivors_n"]:
+        n = int(sv)
+        idx, tt = plan.n_to_idx(n)
+        assert tt == t
+        lo = max(0, idx - (1 << 23))
+        hi = min(plan.R, lo + (1 << 24))
+        say(f"case k={k} {role} t={t} survivor idx={idx} window=[{lo},{hi})")
+        ids, mn = eng.scan_window(t, lo, hi)
+        inside = sorted(i for i in (plan.n_to_idx(int(x))[0] for x in audit["survivors_n"])
+                        if lo <= i < hi)
+        assert sorted(ids) == inside
+        cnt, want = O.c_scan(k, plan.wheel, plan.rest, t, lo, hi, threads=16)
+        assert sorted(ids) == [int(x) for x in want]
+
+
+@pytest.mark.parametrize("k", [270])
+def test_full_block_known_answer(k):
+    """Block 0 of the default-planner wheel: recorded survivor count and g
+    (evidence/logs/2026-08-22/logs/gate_v9_270_280.log)."""
+    rec = json.load(open(os.path.join(GOLDEN, "gate_v9_270_280.json")))[str(k)]
+    eng, plan = engine(k, rec["plan"]["wheel"])
+    assert plan.R == rec["residues_checked"]
+    say(f"case full block k={k} R={plan.R}")
+    ids, mn = eng.scan_window(0, 0, plan.R)
+    ns = sorted(plan.idx_to_n(i, 0) for i in ids)
+    assert len(ns) == rec["survivors_in_block"]
+    assert all(O.is_good(n, k) or n <= k + 1 for n in ns)
+    good = [n for n in ns if O.is_good(n, k)]
+    assert good[0] == int(rec["g"]) == mn

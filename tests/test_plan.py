@@ -1,0 +1,120 @@
+"""Host-side plan logic of the engine against the oracle and the reference's recorded plans."""
+from __future__ import annotations
+
+import json
+import math
+import os
+import random
+
+import pytest
+
+import es_oracle as O
+import es_plan as EP
+from conftest import GOLDEN
+
+
+def test_number_theory_matches_oracle():
+    for k in (1, 2, 30, 97, 200, 399):
+        assert EP.primes_upto(k) == O.primes_upto(k)
+        for p in EP.primes_upto(k):
+            assert EP.allowed_residues(k, p) == O.allowed_residues(k, p)
+            for T in range(1, len(O.digits(k, p)) + 1):
+                assert EP.partial_residues(k, p, T) == O.low_residues(k, p, T)
+    rng = random.Random(3)
+    for _ in range(300):
+        k = rng.randrange(2, 401)
+        n = rng.randrange(1 << 110)
+        p = rng.choice(O.primes_upto(k))
+        assert EP.kummer_ok(n, k, p) == O.dominates(n, k, p)
+
+
+def test_default_plans_k270_275_280():
+    """Default planner = the wheels of evidence/logs/2026-08-22/logs/gate_v9_270_280.log."""
+    rec = json.load(open(os.path.join(GOLDEN, "gate_v9_270_280.json")))
+    for k in (270, 275, 280):
+        plan = EP.Plan(k)
+        s = plan.summary()
+        want = rec[str(k)]["plan"]
+        for key in ("wheel", "M", "residues_per_block", "n_filters", "split_primes",
+                    "density_wheel", "density_rest", "expected_g", "expected_checks_true"):
+            assert s[key] == want[key], (k, key)
+
+
+def test_recorded_frontier_plans():
+    """Every recorded primary plan (default planner) and every banned-prime confirmation plan
+    (harness/run_k_masks.py:250-258) is reproduced exactly, float summary fields included."""
+    plans = json.load(open(os.path.join(GOLDEN, "frontier_plans.json")))
+    checked = set()
+    for x in plans:
+        if x["tag"] not in ("primary", "confirm") or x["log"].endswith("k392_confirm.log"):
+            continue                      # other planners (target_g, v10) or manual wheels
+        k, rec = x["k"], x["plan"]
+        key = (k, json.dumps(rec["wheel"]))
+        if key in checked:
+            continue
+        checked.add(key)
+        if x["banned"]:
+            ban = [int(b) for b in x["banned"].split(",")]
+            full = EP.Plan(k)
+            ps = [p for p in EP.primes_upto(k) if p not in ban]
+            wh = EP.plan_wheel(k, ps, -math.log2(full.density_all), dict(EP.TUNABLES))
+            plan = EP.Plan(k, wheel=[[p, T] for p, T in wh])
+        else:
+            plan = EP.Plan(k)
+        s = plan.summary()
+        for f in ("wheel", "M", "residues_per_block", "n_filters", "density_wheel",
+                  "density_rest", "expected_g"):
+            assert s[f] == rec[f], (k, x["tag"], f)
+    assert len(checked) >= 35
+
+
+def test_runs_json_wheels_geometry():
+    terms = json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
+    for term in terms:
+        for role in ("primary", "confirmation"):
+            r = term[role]
+            plan = EP.Plan(term["k"], None, r["wheel"])
+            assert str(plan.M) == r["N"] and plan.R == r["residues_per_block"]
+            assert [[p, T] for p, T in plan.wheel] == r["wheel"]   # recorded in ring order
+            n = int(term["n"])
+            assert n // plan.M == r["block"]
+
+
+def test_idx_maps_against_oracle():
+    for k, wheel in ((30, [[2, 5], [3, 4], [5, 3], [7, 2], [11, 2], [13, 2]]), (200, None), (399, None)):
+        plan = EP.Plan(k, None, wheel)
+        wm = O.WheelMap(k, plan.wheel)
+        assert wm.M == plan.M and wm.R == plan.R
+        rng = random.Random(k)
+        for _ in range(300):
+            idx, t = rng.randrange(plan.R), rng.randrange(1 << 20)
+            n = plan.idx_to_n(idx, t)
+            assert n == wm.idx_to_n(idx, t)
+            assert plan.n_to_idx(n) == (idx, t)
+
+
+def test_expected_filter_evals_k399():
+    """E_f at the k=399 primary wheel: 1.2176 (reference's measured ~1.22)."""
+    plan = EP.Plan(399, None, [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1],
+                               [137, 1], [41, 1], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1],
+                               [31, 1], [67, 1], [5, 2]])
+    assert abs(plan.expected_filter_evals() - 1.2176) < 1e-4
+    assert plan.radix[0] == 29 and plan.R == 11908694063185920
+
+
+@pytest.mark.parametrize("k", [30, 100, 200, 281, 399])
+def test_masks_are_necessary_conditions(k):
+    """Every word-mask prefilter accepts every residue the full filter accepts, and the slow
+    list plus the masks cover every filter condition."""
+    import gpu_sieve_mi355x as G
+    plan = EP.Plan(k)
+    masks = G.choose_masks(plan, 12)
+    for fi, D, m, rate in masks:
+        p = plan.rest[fi]
+        q, full = plan.info[p]
+        _, low = EP.partial_residues(k, p, D)
+        assert q % m == 0 and set(r % m for r in full) <= set(low)
+        assert 0 < rate <= 1
+    slow = G.slow_order(plan, masks)
+    covered = set(slow) | {fi for fi, D, _, _ in masks if D == plan.tp[plan.rest[fi]]}
+    assert covered == set(range(len(plan.rest)))

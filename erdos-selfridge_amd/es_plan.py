@@ -43,7 +43,9 @@ TUNABLES = {
     "fast_filters": 4,         # filters evaluated as ring-0 word masks
     "word_masks": "auto",      # "auto" | "force" | "off"  (accepted; masks are always exact)
     "mi_launch_log2": 36,      # positions per engine launch (outer units = 2^this / r0)
-    "mi_strip_target": 512,    # target outer indices per uniform strip
+    "mi_strip_target": 1024,   # target outer indices per uniform strip
+    "mi_masks": 8,             # word-mask prefilters staged in LDS
+    "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
 }
 
 

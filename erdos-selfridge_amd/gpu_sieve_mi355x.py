@@ -44,27 +44,40 @@ def pack_bits(q: int, residues) -> np.ndarray:
     return words
 
 
-def choose_masks(plan: Plan, n_masks: int, limit: int | None = None):
-    """Word-mask prefilters: for each filter prime p take the deepest low-digit condition
-    mod p^D (T_p < D <= t_p, p^D <= MASK_MOD_CAP) -- a necessary condition of the full test --
-    and keep the n_masks with the smallest conditional pass rate.
+def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
+                 lds_budget: int = 65536):
+    """Word-mask prefilters: low-digit conditions mod p^D (T_p < D <= t_p, p^D <= MASK_MOD_CAP)
+    of the filter primes -- each a necessary condition of its full test.  Candidates are taken
+    by least conditional pass rate, one depth per prime, while their LDS rows (each table held
+    twice over, two words per ring-0 word per row) fit the budget beside the ring-0 carry table.
     Returns [(filter_index, D, modulus, conditional pass rate)]."""
     k = plan.k
     rest = plan.rest if limit is None else plan.rest[:limit]
+    r0 = plan.radix[0] if plan.radix else 1
+    q0 = plan.rings[0][0] if plan.rings else 1
+    words = (r0 + 31) // 32
     cands = []
     for fi, p in enumerate(rest):
         T = plan.wheelT.get(p, 0)
         base = partial_count(k, p, T) / p ** T if T else 1.0
-        D = 0
         for d in range(T + 1, plan.tp[p] + 1):
-            if p ** d <= MASK_MOD_CAP:
-                D = d
-        if D == 0:
+            if p ** d > MASK_MOD_CAP:
+                break
+            rate = (partial_count(k, p, d) / p ** d) / base
+            cands.append((rate, fi, d, p ** d))
+    cands.sort(key=lambda c: (c[0], c[1], -c[2]))
+    left = lds_budget // 4 - q0 * words - 8
+    taken, out = set(), []
+    for rate, fi, d, m in cands:
+        if len(out) >= n_masks:
+            break
+        cost = 4 * words * m + 4      # rows [0, 2m), two words per ring-0 word
+        if fi in taken or cost > left:
             continue
-        rate = (partial_count(k, p, D) / p ** D) / base
-        cands.append((rate, fi, D, p ** D))
-    cands.sort(key=lambda c: (c[0], c[1]))
-    return [(fi, D, m, rate) for rate, fi, D, m in cands[:max(0, n_masks)]]
+        taken.add(fi)
+        left -= cost
+        out.append((fi, d, m, rate))
+    return out
 
 
 def slow_order(plan: Plan, masks, limit: int | None = None) -> list[int]:
@@ -104,9 +117,11 @@ class _Desc:
         self.term_hi = np.asarray([t >> 64 for t in terms], dtype=np.uint64)
         limit = tun.get("mi_filter_limit")
         limit = None if limit is None else int(limit)
+        # test-only knob: apply just the first `limit` filters (dense survivor sets)
+        self.filters = list(plan.rest if limit is None else plan.rest[:limit])
         qs, offs, words = [], [], []
         nw = 0
-        for p in plan.rest:
+        for p in self.filters:
             q, res = plan.info[p]
             qs.append(q)
             offs.append(nw)
@@ -117,7 +132,7 @@ class _Desc:
         self.filt_off = np.asarray(offs or [0], dtype=np.uint32)
         self.filt_bits = np.concatenate(words) if words else np.zeros(1, dtype=np.uint32)
         n_masks = min(MAX_MASKS, int(tun.get("mi_masks", 8)))
-        self.masks = choose_masks(plan, n_masks, limit)
+        self.masks = choose_masks(plan, n_masks, limit, int(tun.get("mi_lds_budget", 65536)))
         self.slow = slow_order(plan, self.masks, limit)
         mm, ms, mo, mw = [], [], [], []
         nw = 0
@@ -158,7 +173,7 @@ class _Desc:
         d.mask_off = nat.ptr32(self.mask_off)
         d.mask_bits = nat.ptr32(self.mask_bits)
         d.n_mask_words = int(self.mask_bits.size)
-        d.strip_target = int(tun.get("mi_strip_target", 512))
+        d.strip_target = int(tun.get("mi_strip_target", 1024))
         d.threads = int(tun.get("threads", 256))
         d.out_cap = int(tun.get("max_survivors", 1 << 20))
         d.lds_budget = int(tun.get("mi_lds_budget", 65536))
@@ -237,9 +252,9 @@ class GpuSieve:
         nat.check(L.es_engine_info_get(h, C.byref(info)))
         self.info = info
         self.w = len(P.wheel)
-        self.rest_q = [P.info[p][0] for p in P.rest]
+        self.rest_q = [P.info[p][0] for p in self._d.filters]
         self.rest_Mmodq = [P.M % q for q in self.rest_q]
-        self.nrest = len(P.rest)
+        self.nrest = len(self.rest_q)
         self.tconst_h = np.zeros(max(1, self.nrest), dtype=np.uint32)
         self.tconst = _TConst(self)
         self.out_cap = int(P.tun["max_survivors"])

@@ -81,8 +81,9 @@ struct HotPlan {
   const uint32_t* base_off;        // [nb] first term of each base ring
   const uint32_t* term_h;          // base-ring terms, h part
   const uint64_t* term_l;          // base-ring terms, l part (lo, hi pairs)
-  const uint32_t* masks;           // [F][8] m, LDS byte base, M0 mod m, 2^32, 2^64, 2^96 mod m,
+  const uint32_t* masks;           // [F][8] m, table byte base, M0 mod m, 2^32, 2^64, 2^96 mod m,
                                    //        (-M) mod m, t*M mod m
+  const uint32_t* gmask;           // mask tables served from global memory (through L1)
   const ColdPlan* cold;
 };
 
@@ -128,7 +129,9 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
 }
 
 // One strip: S outer indices, uniform across the wave.  W = words per ring-0 mask (0 = runtime).
-template <int F, int W, bool CHECK>
+// The last G of the F masks are read from global memory (vector loads through L1) instead of
+// LDS, so the two load pipes share the row traffic.
+template <int F, int G, int W, bool CHECK>
 __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
                                          const uint32_t (&A0)[F + 1],
                                          const uint32_t (&A1)[F + 1], uint32_t hu, uint64_t zl,
@@ -140,6 +143,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
   const uint32_t q0 = P.q0, S = P.S, stride = P.stride;
   const uint32_t nw = W ? (uint32_t)W : P.W;
   const char* lb = reinterpret_cast<const char*>(lds);
+  const char* gb = reinterpret_cast<const char*>(P.gmask);
   const uint32_t* carry_tab = reinterpret_cast<const uint32_t*>(lb + P.carry_base);
   for (uint32_t sg = 0; sg < S; ++sg) {
     cu32_t* row = (cu32_t*)(P.strip) + (size_t)sg * stride;
@@ -158,7 +162,8 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         const uint32_t a = (c1 ? A1[f] : A0[f]) + row[8 + f];
-        const uint2 v = *reinterpret_cast<const uint2*>(lb + a);
+        const uint2 v = f < F - G ? *reinterpret_cast<const uint2*>(lb + a)
+                                  : *reinterpret_cast<const uint2*>(gb + a);
         bits &= (carry & v.y) | (~carry & v.x);
       }
       if (!live) bits = 0;
@@ -208,7 +213,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
   }
 }
 
-template <int F, int W>
+template <int F, int G, int W>
 __global__ void __launch_bounds__(1024)
 es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, uint64_t o_hi,
                 uint64_t idx_lo, uint64_t idx_hi, unsigned long long* __restrict__ out,
@@ -282,39 +287,46 @@ es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, ui
     }
     const bool full = in && sb == 0 && se == S;
     if (__all(full))
-      es_sweep<F, W, false>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out, out_n,
-                            out_cap);
+      es_sweep<F, G, W, false>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out,
+                               out_n, out_cap);
     else
-      es_sweep<F, W, true>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out, out_n,
-                           out_cap);
+      es_sweep<F, G, W, true>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out,
+                              out_n, out_cap);
   }
 }
 
 typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t,
                          unsigned long long*, unsigned long long*, uint64_t);
 
+// Instances: every F <= 12 with all masks in LDS for W = 1, 2 and runtime W; for W = 1 also
+// G = 2 or 4 of the masks in global memory.
 template <int F>
-KernelFn kernel_for_w(uint32_t W) {
-  if (W == 1) return es_sieve_kernel<F, 1>;
-  if (W == 2) return es_sieve_kernel<F, 2>;
-  return es_sieve_kernel<F, 0>;
+KernelFn kernel_for(uint32_t W, uint32_t G) {
+  if (W == 1) {
+    if (G == 2 && F >= 2) return es_sieve_kernel<F, (F >= 2 ? 2 : 0), 1>;
+    if (G == 4 && F >= 4) return es_sieve_kernel<F, (F >= 4 ? 4 : 0), 1>;
+    return G == 0 ? es_sieve_kernel<F, 0, 1> : nullptr;
+  }
+  if (G != 0) return nullptr;
+  if (W == 2) return es_sieve_kernel<F, 0, 2>;
+  return es_sieve_kernel<F, 0, 0>;
 }
 
-KernelFn pick_kernel(int F, uint32_t W) {
+KernelFn pick_kernel(int F, uint32_t W, uint32_t G) {
   switch (F) {
-    case 0: return kernel_for_w<0>(W);
-    case 1: return kernel_for_w<1>(W);
-    case 2: return kernel_for_w<2>(W);
-    case 3: return kernel_for_w<3>(W);
-    case 4: return kernel_for_w<4>(W);
-    case 5: return kernel_for_w<5>(W);
-    case 6: return kernel_for_w<6>(W);
-    case 7: return kernel_for_w<7>(W);
-    case 8: return kernel_for_w<8>(W);
-    case 9: return kernel_for_w<9>(W);
-    case 10: return kernel_for_w<10>(W);
-    case 11: return kernel_for_w<11>(W);
-    case 12: return kernel_for_w<12>(W);
+    case 0: return kernel_for<0>(W, G);
+    case 1: return kernel_for<1>(W, G);
+    case 2: return kernel_for<2>(W, G);
+    case 3: return kernel_for<3>(W, G);
+    case 4: return kernel_for<4>(W, G);
+    case 5: return kernel_for<5>(W, G);
+    case 6: return kernel_for<6>(W, G);
+    case 7: return kernel_for<7>(W, G);
+    case 8: return kernel_for<8>(W, G);
+    case 9: return kernel_for<9>(W, G);
+    case 10: return kernel_for<10>(W, G);
+    case 11: return kernel_for<11>(W, G);
+    case 12: return kernel_for<12>(W, G);
     default: return nullptr;
   }
 }
@@ -334,7 +346,7 @@ struct Blob {
 struct es_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint32_t k = 0, w = 0, r0 = 1, q0 = 1, W = 1, S = 1, F = 0, threads = 256;
+  uint32_t k = 0, w = 0, r0 = 1, q0 = 1, W = 1, S = 1, F = 0, G = 0, threads = 256;
   uint32_t n_filters = 0, n_slow = 0, lds_bytes = 0, max_grid = 1;
   u128 M = 1, M0 = 1, R = 1;
   uint64_t R_outer = 1, strips = 1, out_cap = 1u << 20, launch_outer = 1;
@@ -522,16 +534,44 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->strips = e->R_outer / e->S;
   const uint32_t nb = w - 1 - m;
 
-  // ---- LDS image: doubled mask tables (rows of 2W words), then the carry table
+  // ---- mask tables: rows [0, 2m) of 2W words (c0, c1 per ring-0 word).  Staged masks go to
+  // LDS, except that for W = 1 the G smallest tables are read from global memory through L1.
   const uint32_t budget_words = (d->lds_budget ? d->lds_budget : 65536u) / 4;
   const uint32_t carry_words = q0 * W;
-  std::vector<uint32_t> lds, mbase, mmod, msrc;
-  for (uint32_t f = 0; f < d->n_masks && mbase.size() < (size_t)kMaxMasks; ++f) {
+  std::vector<uint32_t> staged;  // indices into the descriptor's mask list
+  {
+    size_t used = carry_words + 8;
+    for (uint32_t f = 0; f < d->n_masks && staged.size() < (size_t)kMaxMasks; ++f) {
+      const size_t rows_words = (size_t)4 * d->mask_mod[f] * W + 4;
+      if (used + rows_words > budget_words) continue;  // stays exact-only
+      used += rows_words;
+      staged.push_back(f);
+    }
+  }
+  uint32_t G = d->global_masks;
+  if (W != 1 || (G != 2 && G != 4) || staged.size() < G) G = 0;
+  // the G smallest staged tables go global, kept in their relative order at the end
+  std::vector<uint32_t> glob;
+  if (G) {
+    std::vector<uint32_t> bysize(staged);
+    std::stable_sort(bysize.begin(), bysize.end(),
+                     [&](uint32_t a, uint32_t b) { return d->mask_mod[a] < d->mask_mod[b]; });
+    glob.assign(bysize.begin(), bysize.begin() + G);
+  }
+  std::vector<uint32_t> order_masks;
+  for (uint32_t f : staged)
+    if (std::find(glob.begin(), glob.end(), f) == glob.end()) order_masks.push_back(f);
+  for (uint32_t f : staged)
+    if (std::find(glob.begin(), glob.end(), f) != glob.end()) order_masks.push_back(f);
+  std::vector<uint32_t> lds, gimg, mbase, mmod, msrc;
+  for (size_t pos = 0; pos < order_masks.size(); ++pos) {
+    const uint32_t f = order_masks[pos];
+    const bool in_global = pos + G >= order_masks.size();
+    std::vector<uint32_t>& img = in_global ? gimg : lds;
     const uint32_t mq = d->mask_mod[f], src = d->mask_src[f];
     const size_t rows_words = (size_t)2 * mq * 2 * W;
-    const size_t off = (lds.size() + 3) & ~size_t(3);  // 16-byte aligned rows
-    if (off + rows_words + carry_words + 4 > budget_words) continue;  // stays exact-only
-    lds.resize(off + rows_words, 0);
+    const size_t off = (img.size() + 3) & ~size_t(3);  // 16-byte aligned rows
+    img.resize(off + rows_words, 0);
     const uint32_t* bm = d->mask_bits + d->mask_off[f];
     const uint32_t corr = (mq - mod_u128(M, mq)) % mq;
     std::vector<uint32_t> t0m(r0);
@@ -542,7 +582,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
         const uint32_t b = (uint32_t)(((uint64_t)a + corr) % mq);
         const uint32_t wi = i >> 5, bit = 1u << (i & 31);
         for (uint32_t copy = 0; copy < 2; ++copy) {
-          uint32_t* row = &lds[off + (size_t)2 * W * (x + copy * mq) + 2 * wi];
+          uint32_t* row = &img[off + (size_t)2 * W * (x + copy * mq) + 2 * wi];
           if ((bm[a >> 5] >> (a & 31)) & 1u) row[0] |= bit;
           if ((bm[b >> 5] >> (b & 31)) & 1u) row[1] |= bit;
         }
@@ -552,7 +592,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     mmod.push_back(mq);
     msrc.push_back(src);
   }
+  if (gimg.empty()) gimg.push_back(0);
   e->F = (uint32_t)mmod.size();
+  e->G = G;
   e->mask_mod = mmod;
   e->mask_src = msrc;
   const uint32_t carry_off = (uint32_t)((lds.size() + 3) & ~size_t(3));
@@ -661,6 +703,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // ---- device upload: one blob
   Blob bl;
   const size_t o_lds = bl.put(lds.data(), lds.size() * 4);
+  const size_t o_gm = bl.put(gimg.data(), gimg.size() * 4);
   const size_t o_strip = bl.put(strip.data(), strip.size() * 4);
   const size_t o_brad = bl.put(b_radix.data(), b_radix.size() * 4);
   const size_t o_boff = bl.put(b_off.data(), b_off.size() * 4);
@@ -679,7 +722,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
   if (e->lds_bytes > prop.sharedMemPerBlock)
     return fail(ES_ERR_LIMIT, "LDS tables exceed the device's per-workgroup LDS");
-  e->max_grid = (uint32_t)std::max(1, prop.multiProcessorCount) * (2048u / e->threads) * 2u;
+  e->max_grid = 1u << 30;  // one strip per thread; the dispatcher balances the tail
   ES_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   ES_HIP(hipMalloc(&e->dblob, bl.bytes.size()));
   uint8_t* base = (uint8_t*)e->dblob;
@@ -720,13 +763,14 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   P.term_h = (const uint32_t*)(base + o_th);
   P.term_l = (const uint64_t*)(base + o_tl);
   P.masks = (const uint32_t*)(base + o_mask);
+  P.gmask = (const uint32_t*)(base + o_gm);
   P.cold = (const ColdPlan*)(base + o_cold);
-  e->kern = pick_kernel((int)F, W);
+  e->kern = pick_kernel((int)F, W, G);
   if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
   if (e->lds_bytes > 65536)
     ES_HIP(hipFuncSetAttribute((const void*)e->kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)e->lds_bytes));
-  e->launch_outer = std::max<uint64_t>(1, ((uint64_t)1 << 36) / r0);
+  e->launch_outer = std::max<uint64_t>(1, ((uint64_t)1 << (d->launch_log2 ? d->launch_log2 : 37)) / r0);
   e->ev_pool.resize(512);
   for (auto& ev : e->ev_pool) ES_HIP(hipEventCreate(&ev));
   ES_HIP(hipMemsetAsync(e->d_count, 0, 8, e->stream));
@@ -758,6 +802,7 @@ int es_engine_info_get(const es_engine* e, es_engine_info* info) {
   info->q0 = e->q0;
   info->strip_len = e->S;
   info->n_masks_used = e->F;
+  info->global_masks = e->G;
   info->lds_bytes = e->lds_bytes;
   info->words = e->W;
   info->threads = e->threads;
@@ -797,8 +842,11 @@ int es_engine_scan_window(es_engine* e, uint64_t idx_lo, uint64_t idx_hi) {
   if (idx_hi > (uint64_t)e->R || idx_lo > idx_hi) return fail(ES_ERR_INVALID, "window outside [0, R)");
   if (idx_lo == idx_hi) return ES_OK;
   const uint64_t o_lo = idx_lo / e->r0, o_hi = (idx_hi + e->r0 - 1) / e->r0;
-  for (uint64_t a = o_lo; a < o_hi; a += e->launch_outer) {
-    const uint64_t b = std::min(o_hi, a + e->launch_outer);
+  // equal launches of at most launch_outer outer indices (no short tail launch)
+  const uint64_t n = (o_hi - o_lo + e->launch_outer - 1) / e->launch_outer;
+  const uint64_t per = (o_hi - o_lo + n - 1) / n;
+  for (uint64_t a = o_lo; a < o_hi; a += per) {
+    const uint64_t b = std::min(o_hi, a + per);
     const int rc = launch_range(e, a, b, idx_lo, idx_hi);
     if (rc) return rc;
   }

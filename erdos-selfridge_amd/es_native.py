@@ -28,7 +28,8 @@ class PlanDesc(C.Structure):
         ("n_masks", C.c_uint32), ("mask_mod", u32p), ("mask_src", u32p), ("mask_off", u32p),
         ("mask_bits", u32p), ("n_mask_words", C.c_uint32),
         ("strip_target", C.c_uint32), ("threads", C.c_uint32), ("out_cap", C.c_uint64),
-        ("lds_budget", C.c_uint32), ("reserved", C.c_uint32),
+        ("lds_budget", C.c_uint32), ("global_masks", C.c_uint32), ("launch_log2", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -38,6 +39,7 @@ class EngineInfo(C.Structure):
         ("strips", C.c_uint64), ("r0", C.c_uint32), ("q0", C.c_uint32),
         ("strip_len", C.c_uint32), ("n_masks_used", C.c_uint32), ("lds_bytes", C.c_uint32),
         ("words", C.c_uint32), ("threads", C.c_uint32), ("device", C.c_uint32),
+        ("global_masks", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -70,6 +72,16 @@ def lib() -> C.CDLL:
         "es_engine_collect": (C.c_int, [eng, u64p, C.c_uint64, u64p, u64p, C.POINTER(C.c_int)]),
         "es_engine_sync": (C.c_int, [eng]),
         "es_engine_kernel_time": (C.c_int, [eng, C.POINTER(C.c_double), u64p, u64p]),
+        "es_fpass_create": (C.c_int, [C.c_uint32, u32p, u32p, u64p, C.c_uint32, C.c_uint32,
+                                      C.c_uint64, C.POINTER(eng)]),
+        "es_fpass_destroy": (C.c_int, [eng]),
+        "es_fpass_upload": (C.c_int, [eng, u64p, C.c_uint64, u32p, C.c_uint64]),
+        "es_fpass_generate": (C.c_int, [eng, C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p,
+                                        u32p, C.c_uint64]),
+        "es_fpass_run": (C.c_int, [eng, u64p]),
+        "es_fpass_read": (C.c_int, [eng, u32p, C.c_uint64]),
+        "es_fpass_read_stream": (C.c_int, [eng, C.c_uint64, C.c_uint64, u64p, u32p]),
+        "es_fpass_last_ms": (C.c_int, [eng, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -42,10 +42,11 @@ TUNABLES = {
     "max_survivors": 1 << 20,  # survivor buffer capacity (u64 indices)
     "fast_filters": 4,         # filters evaluated as ring-0 word masks
     "word_masks": "auto",      # "auto" | "force" | "off"  (accepted; masks are always exact)
-    "mi_launch_log2": 36,      # positions per engine launch (outer units = 2^this / r0)
+    "mi_launch_log2": 37,      # positions per engine launch (outer units = 2^this / r0)
     "mi_strip_target": 1024,   # target outer indices per uniform strip
     "mi_masks": 8,             # word-mask prefilters staged in LDS
     "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
+    "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
 }
 
 

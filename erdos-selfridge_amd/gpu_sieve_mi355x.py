@@ -177,6 +177,8 @@ class _Desc:
         d.threads = int(tun.get("threads", 256))
         d.out_cap = int(tun.get("max_survivors", 1 << 20))
         d.lds_budget = int(tun.get("mi_lds_budget", 65536))
+        d.global_masks = int(tun.get("mi_global_masks", 0))
+        d.launch_log2 = int(tun.get("mi_launch_log2", 37))
         self.desc = d
 
 
@@ -263,12 +265,12 @@ class GpuSieve:
         self.r0 = P.radix[0] if P.radix else 1
         self.n_fast = info.n_masks_used
         self.threads = int(P.tun["threads"])
-        self.chunk = 1 << int(P.tun.get("mi_launch_log2", 36))
+        self.chunk = 1 << int(P.tun.get("mi_launch_log2", 37))
         self.R_outer = P.R // self.r0
         assert self.R_outer == info.R_outer and self.R_outer * self.r0 == P.R
         self.ochunk = max(1, self.chunk // self.r0)
         self.strip_len = info.strip_len
-        self.mask_mode = f"mi355x-lds-mask{info.n_masks_used}"
+        self.mask_mode = f"mi355x-mask{info.n_masks_used}-l1{info.global_masks}"
         self.masks = self._d.masks
         self.launches = 0
         self.residues_checked = 0

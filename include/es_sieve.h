@@ -69,6 +69,8 @@ typedef struct es_plan_desc {
   uint32_t threads;            /* workgroup size: 64, 128, 256, 512 or 1024 (0 = 256) */
   uint64_t out_cap;            /* survivor buffer capacity in u64 IDs (0 = 2^20) */
   uint32_t lds_budget;         /* max bytes of LDS tables per workgroup (0 = 64 KiB) */
+  uint32_t global_masks;       /* masks read through L1 instead of LDS: 0, 2 or 4 (r0 <= 32) */
+  uint32_t launch_log2;        /* positions per kernel launch, log2 (0 = 37) */
   uint32_t reserved;
 } es_plan_desc;
 
@@ -83,6 +85,8 @@ typedef struct es_engine_info {
   uint32_t words;              /* u32 words per ring-0 mask row */
   uint32_t threads;
   uint32_t device;
+  uint32_t global_masks;       /* masks read through L1 */
+  uint32_t reserved;
 } es_engine_info;
 
 /* Last error message of the calling thread ("" if none).  Never NULL. */
@@ -124,6 +128,36 @@ int es_engine_sync(es_engine* e);
 /* Device time of the sieve kernels launched since the last call (HIP events on the engine's
  * stream), their launch count, and the positions they covered. */
 int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t* positions);
+
+/* ---- Candidate filter pass over a 128-bit value stream (HBM-bound) ------------------------
+ * Reference interface replaced: the late scalar filters the word-mask kernel runs on the
+ * candidates its masks leave, emitting original IDs (sieve/es_word_masks.py:87-92 with the
+ * per-filter test of sieve/gpu_sieve_v8.py:316-322).  Here the candidates are an explicit
+ * list of u32 IDs into a stream of u128 values resident in HBM; every ID whose value passes
+ * every filter table (bit n mod q_j set) is emitted, in unspecified order. */
+typedef struct es_fpass es_fpass;
+
+/* Filters in test order (most selective first); tables are u64 words, bit r = residue r. */
+int es_fpass_create(uint32_t n_filters, const uint32_t* filt_q, const uint32_t* filt_off,
+                    const uint64_t* words, uint32_t n_words, uint32_t lds_budget,
+                    uint64_t out_cap, es_fpass** out);
+int es_fpass_destroy(es_fpass* f);
+/* Copy a host stream (values as lo, hi u64 pairs) and candidate IDs to the device. */
+int es_fpass_upload(es_fpass* f, const uint64_t* values_lohi, uint64_t n_values,
+                    const uint32_t* ids, uint64_t n_ids);
+/* Fill a stream of n values on the device from a seed: each value is
+ * (CRT combination of one random admissible residue per conditioned modulus) + J * prod(mods)
+ * with J uniform in [0, j_span); candidate IDs are 0..n-1 permuted inside tiles of 64. */
+int es_fpass_generate(es_fpass* f, uint64_t n, uint64_t seed, uint32_t n_mods,
+                      const uint32_t* mods, const uint32_t* res_count, const uint32_t* residues,
+                      uint64_t j_span);
+/* Run the pass (synchronous); *n_out = survivors (may exceed out_cap: ES_ERR_OVERFLOW). */
+int es_fpass_run(es_fpass* f, uint64_t* n_out);
+int es_fpass_read(es_fpass* f, uint32_t* ids, uint64_t count);
+int es_fpass_read_stream(es_fpass* f, uint64_t offset, uint64_t count, uint64_t* values_lohi,
+                         uint32_t* ids);
+/* Device time of the last es_fpass_run (HIP events on the pass's stream). */
+int es_fpass_last_ms(es_fpass* f, double* ms);
 
 #ifdef __cplusplus
 }

@@ -230,3 +230,26 @@ int eso_dominates(uint64_t n_lo, uint64_t n_hi, uint32_t k, uint32_t p) {
   }
   return 1;
 }
+
+/* Candidate filter pass restated (sieve/es_word_masks.py:87-88, sieve/gpu_sieve_v8.py:316-322):
+ * every candidate ID whose value n dominates k in base p for every listed prime, in input order.
+ * Returns the survivor count (only cap IDs are written). */
+int64_t eso_filter_values(uint32_t k, uint32_t nf, const uint32_t* primes,
+                          const uint64_t* values_lohi, uint64_t n_values, const uint32_t* ids,
+                          uint64_t n_ids, uint32_t* out, uint64_t cap) {
+  uint64_t qs[ESO_MAX_FILT];
+  if (nf > ESO_MAX_FILT) return -1;
+  for (uint32_t j = 0; j < nf; ++j) qs[j] = ipow(primes[j], ndigits(k, primes[j]));
+  uint64_t n = 0;
+  for (uint64_t i = 0; i < n_ids; ++i) {
+    if (ids[i] >= n_values) return -1;
+    const u128 v = ((u128)values_lohi[2 * (uint64_t)ids[i] + 1] << 64) | values_lohi[2 * (uint64_t)ids[i]];
+    int ok = 1;
+    for (uint32_t j = 0; j < nf && ok; ++j) ok = dominates_r((uint64_t)(v % qs[j]), k, primes[j]);
+    if (ok) {
+      if (n < cap) out[n] = ids[i];
+      ++n;
+    }
+  }
+  return (int64_t)n;
+}

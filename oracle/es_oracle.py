@@ -193,6 +193,9 @@ def lib():
         L.eso_idx_to_s.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint64, u64p]
         L.eso_dominates.restype = C.c_int
         L.eso_dominates.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.eso_filter_values.restype = C.c_int64
+        L.eso_filter_values.argtypes = [C.c_uint32, C.c_uint32, u32p, u64p, C.c_uint64, u32p,
+                                        C.c_uint64, u32p, C.c_uint64]
         _lib = L
     return _lib
 
@@ -242,3 +245,23 @@ def c_idx_to_n(k: int, wheel_ring_order, idx: int, t: int) -> int:
         raise ValueError("oracle rejected the plan")
     M, _, _ = c_geometry(k, wheel_ring_order)
     return t * M + ((int(s[1]) << 64) | int(s[0]))
+
+
+def c_filter_values(k: int, primes, values, ids, cap: int = 1 << 22):
+    """Survivor IDs (input order) of the candidate filter pass: ids whose value dominates k in
+    base p for every listed prime."""
+    import numpy as np
+    L = lib()
+    v = np.zeros(2 * max(1, len(values)), dtype=np.uint64)
+    for i, x in enumerate(values):
+        v[2 * i] = x & ((1 << 64) - 1)
+        v[2 * i + 1] = x >> 64
+    ids = np.ascontiguousarray(np.asarray(ids, dtype=np.uint32))
+    out = np.zeros(max(1, cap), dtype=np.uint32)
+    u32p, u64p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+    n = L.eso_filter_values(k, len(primes), _u32(list(primes)), v.ctypes.data_as(u64p),
+                            len(values), ids.ctypes.data_as(u32p), ids.size,
+                            out.ctypes.data_as(u32p), cap)
+    if n < 0:
+        raise ValueError("oracle rejected the input")
+    return out[:min(n, cap)].copy()

@@ -6,7 +6,7 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "erdos-selfridge_amd")
-for p in (PKG, os.path.join(ROOT, "oracle"), ROOT):
+for p in (PKG, os.path.join(ROOT, "oracle"), ROOT, os.path.join(ROOT, "tests")):
     if p not in sys.path:
         sys.path.insert(0, p)
 

@@ -1,0 +1,142 @@
+"""Candidate filter pass (k = 397, filter primes 101..397): CPU checks of the tables, the
+generator specification and the oracle; GPU parity of the pass against the oracle."""
+from __future__ import annotations
+
+import math
+import random
+
+import numpy as np
+import pytest
+
+import es_filter_pass as FP
+import es_oracle as O
+import es_plan as EP
+
+K = 397
+PRIMES = [p for p in O.primes_upto(K) if p >= 101]
+
+
+def test_filter_tables_match_oracle_residues():
+    order = FP.filter_order(K, PRIMES)
+    assert sorted(order) == PRIMES and len(order) == 53
+    qs, offs, words = FP.filter_tables(K, order[:5])
+    for p, q, off in zip(order[:5], qs, offs):
+        q2, res = O.allowed_residues(K, p)
+        assert q == q2 == p * p
+        got = [r for r in range(q) if (int(words[off + (r >> 6)]) >> (r & 63)) & 1]
+        assert got == res
+
+
+def test_generator_spec_bounds_and_survival():
+    mods, resl, j_span, surv = FP.generator_spec(K, FP.filter_order(K, PRIMES))
+    P = math.prod(mods)
+    assert P < 1 << 108 and P * j_span <= 1 << 113 and j_span >= 16
+    for a in range(len(mods)):
+        for b in range(a + 1, len(mods)):
+            assert math.gcd(mods[a], mods[b]) == 1
+    assert 5e-7 < surv < 1e-5                       # about the most values below 2^113 allow
+    for m, res in zip(mods, resl):
+        p = min(x for x in PRIMES if m % x == 0)
+        d = round(math.log(m, p))
+        assert res == O.low_residues(K, p, d)[1]
+
+
+def _spec_values(spec, n, seed):
+    """Python draw of values of a generator spec (the device draw uses another RNG)."""
+    mods, resl, j_span, _ = spec
+    P = math.prod(mods)
+    rng = random.Random(seed)
+    basis = [(P // m) * pow(P // m, -1, m) for m in mods]
+    out = []
+    for _ in range(n):
+        s = sum(rng.choice(r) * b for r, b in zip(resl, basis)) % P
+        out.append(s + rng.randrange(j_span) * P)
+    return out
+
+
+def test_conditional_order_puts_fixed_digits_late():
+    spec = FP.generator_spec(K, FP.filter_order(K, PRIMES))
+    order = FP.conditional_order(K, PRIMES, spec)
+    fixed = {min(x for x in PRIMES if m % x == 0) for m in spec[0]}
+    assert sorted(order) == PRIMES
+    assert not fixed & set(order[:5])
+
+
+def test_generator_spec_survival_monte_carlo():
+    """Dense variant (3 filters, 24-bit budget): measured survival matches the spec."""
+    prim = [101, 103, 107]
+    spec = FP.generator_spec(K, prim, mod_bits=24)
+    vals = _spec_values(spec, 20000, 3)
+    keep = O.c_filter_values(K, prim, vals, np.arange(len(vals)))
+    frac = len(keep) / len(vals)
+    assert abs(frac - spec[3]) < 5 * math.sqrt(spec[3] / len(vals))
+
+
+def test_oracle_filter_matches_python_kummer():
+    rng = random.Random(11)
+    prim = PRIMES[:6]
+    vals = [rng.randrange(1 << 113) for _ in range(3000)]
+    # plant admissible residues for the 6 primes on a third of the values
+    spec = FP.generator_spec(K, prim, mod_bits=90)
+    vals[::3] = _spec_values(spec, len(vals[::3]), 4)
+    ids = np.arange(len(vals), dtype=np.uint32)[::-1].copy()
+    got = O.c_filter_values(K, prim, vals, ids)
+    want = [int(i) for i in ids if all(O.dominates(vals[i], K, p) for p in prim)]
+    assert [int(x) for x in got] == want and len(want) > 100
+
+
+# ------------------------------------------------------------------ GPU parity
+
+@pytest.mark.gpu
+def test_gpu_pass_uploaded_stream_dense():
+    prim = FP.filter_order(K, PRIMES[:8])
+    fp = FP.FilterPass(K, prim)
+    spec = FP.generator_spec(K, fp.primes, mod_bits=100)
+    vals = _spec_values(spec, 1 << 16, 5) + [random.Random(6).randrange(1 << 120) for _ in range(1 << 14)]
+    ids = np.random.default_rng(7).permutation(len(vals)).astype(np.uint32)
+    print(f"case fpass uploaded n={len(vals)} filters={len(prim)}", flush=True)
+    fp.upload(vals, ids)
+    got = fp.run()
+    want = np.sort(O.c_filter_values(K, fp.primes, vals, ids))
+    assert len(want) > 1000 and np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_gpu_pass_generated_stream_all_filters():
+    fp = FP.FilterPass(K)
+    n = 1 << 22
+    print(f"case fpass generated n={n} filters=53", flush=True)
+    fp.generate(n, seed=20260901)
+    got = fp.run()
+    vals, ids = fp.read_stream(0, n)
+    assert sorted(int(i) for i in ids) == list(range(n))        # a permutation of the stream
+    assert all(v < (1 << 113) for v in vals[:4096])
+    want = np.sort(O.c_filter_values(K, fp.primes, vals, ids))
+    assert np.array_equal(got, want)
+    # same seed, same stream
+    fp.generate(n, seed=20260901)
+    v2, _ = fp.read_stream(0, 1024)
+    assert v2 == vals[:1024]
+
+
+@pytest.mark.gpu
+def test_gpu_pass_conditional_order_same_survivors():
+    """Filter order changes speed only: the conditional order returns the same survivors."""
+    spec = FP.generator_spec(K, FP.filter_order(K, PRIMES))
+    a = FP.FilterPass(K)
+    b = FP.FilterPass(K, FP.conditional_order(K, PRIMES, spec), order="given")
+    n = 1 << 20
+    print(f"case fpass orders n={n}", flush=True)
+    a.generate(n, seed=77, spec=spec)
+    b.generate(n, seed=77, spec=spec)
+    assert np.array_equal(a.run(), b.run())
+
+
+@pytest.mark.gpu
+def test_gpu_pass_overflow_reported():
+    prim = [101]
+    fp = FP.FilterPass(K, prim, out_cap=16)
+    vals = [x for x in range(1000)]
+    fp.upload(vals, np.arange(1000))
+    with pytest.raises(RuntimeError, match="overflow"):
+        fp.run()

@@ -41,7 +41,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("sieve", "fpass", "copy"), default="sieve",
+    ap.add_argument("--autotune", action="store_true", help="batch: autotune each k first")
+    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch"), default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
                          "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters)")
     ap.add_argument("--fpass-log2", type=int, default=30)
@@ -54,6 +55,8 @@ def main() -> int:
 
     if a.workload == "fpass":
         return bench_fpass(a)
+    if a.workload == "batch":
+        return bench_batch(a)
     if a.workload == "copy":
         import torch
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")       # 1 GiB
@@ -208,6 +211,69 @@ def bench_fpass(a) -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_candidate": bytes_per, "kernel_avg_launch_ms": dev_ms},
         "cpu_baseline": cpu}), flush=True)
+    return 0
+
+
+def bench_batch(a) -> int:
+    """k = 300..320 (default planner wheels), 4 seeded blocks per k, windows of 2^34 positions
+    at seeded offsets: one batched run (one grid-mapped launch per kernel shape) against the same
+    jobs run one engine launch sequence at a time.  Survivors must agree job by job."""
+    import random
+
+    import es_batch
+    import es_plan
+    import gpu_sieve_mi355x as G
+    rng = random.Random(300320)
+    N = 1 << 34
+    tuned = {}
+    if a.autotune:
+        import es_autotune
+        # batched launches read every mask from LDS: the L1-mask layout is not searched
+        space = dict(es_autotune.SPACE, mi_global_masks=[0])
+        for k in range(300, 321):
+            tuned[k], _ = es_autotune.autotune(k, space=space, probe_log2=34, method="coord")
+    engines = {k: G.GpuSieve(es_plan.Plan(k, tuned.get(k))) for k in range(300, 321)}
+    jobs = []
+    for k, eng in engines.items():
+        for _ in range(4):
+            t = rng.randrange(1 << 20)
+            lo = rng.randrange(eng.plan.R - N)
+            jobs.append((k, t, lo, lo + N))
+    b = es_batch.Batch()
+    ids = [b.add(engines[k], t, lo, hi) for k, t, lo, hi in jobs]
+    b.run()                                              # warm-up
+    times = []
+    for _ in range(max(1, a.steps)):
+        ms, launches = b.run()
+        times.append(ms)
+    bms = min(times)
+    # unbatched: the same jobs, one engine scan (launch sequence) each
+    for k, t, lo, hi in jobs[:4]:
+        engines[k].scan_window(t, lo, hi)
+    for eng in engines.values():
+        eng.sync()
+        eng.kernel_time()
+    import time as _t
+    t0 = _t.perf_counter()
+    single = []
+    for (k, t, lo, hi), jid in zip(jobs, ids):
+        got, _ = engines[k].scan_window(t, lo, hi)
+        single.append(sorted(got))
+    swall = _t.perf_counter() - t0
+    sms = sum(engines[k].kernel_time()[0] for k in engines)
+    for jid, want in zip(ids, single):
+        assert b.collect(jid) == want
+    pos = N * len(jobs)
+    print(json.dumps({
+        "metric": "batched residue positions sieved/s (k=300..320, 4 blocks each)",
+        "value": pos / (bms / 1e3), "unit": "positions/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": 1, "ms_per_step": bms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u128/u32 integer", "data": "synthetic seeded windows",
+        "config": {"workload": "21 plans x 4 blocks x 2^34 positions", "jobs": len(jobs),
+                   "launches": launches, "autotuned": bool(a.autotune),
+                   "tuned": {str(k): v for k, v in tuned.items()}},
+        "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
+        "survivors_total": sum(len(x) for x in single)}), flush=True)
     return 0
 
 

@@ -68,6 +68,9 @@ struct ColdPlan {
   const uint32_t* inner_ids;  // [r0] original ring-0 digit of each sorted slot
   const uint32_t* slow;       // [n_slow][8] q, word offset, 2^32, 2^64, 2^96 mod q, t*M mod q
   const uint32_t* slow_bits;  // admissibility bitmaps
+  unsigned long long* out;    // survivor IDs
+  unsigned long long* out_n;  // survivor counter (may pass out_cap)
+  uint64_t out_cap;
 };
 
 // Kernel argument.
@@ -92,10 +95,7 @@ struct HotPlan {
 __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint32_t bits,
                                            uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
                                            uint64_t zh, uint32_t sg, uint64_t o,
-                                           uint64_t idx_lo, uint64_t idx_hi,
-                                           unsigned long long* __restrict__ out,
-                                           unsigned long long* __restrict__ out_n,
-                                           uint64_t out_cap) {
+                                           uint64_t idx_lo, uint64_t idx_hi) {
   const uint32_t* row = C->strip + (size_t)sg * C->stride;
   const u128 L = es::mk128((uint64_t)row[2] | ((uint64_t)row[3] << 32),
                        (uint64_t)row[4] | ((uint64_t)row[5] << 32));
@@ -122,8 +122,8 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
       ok = (C->slow_bits[f[1] + (r >> 5)] >> (r & 31u)) & 1u;
     }
     if (ok) {
-      const unsigned long long slot = atomicAdd(out_n, 1ull);
-      if (slot < out_cap) out[slot] = idx;
+      const unsigned long long slot = atomicAdd(C->out_n, 1ull);
+      if (slot < C->out_cap) C->out[slot] = idx;
     }
   }
 }
@@ -136,10 +136,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
                                          const uint32_t (&A0)[F + 1],
                                          const uint32_t (&A1)[F + 1], uint32_t hu, uint64_t zl,
                                          uint64_t zh, uint64_t obase, uint32_t sb, uint32_t se,
-                                         uint64_t idx_lo, uint64_t idx_hi,
-                                         unsigned long long* __restrict__ out,
-                                         unsigned long long* __restrict__ out_n,
-                                         uint64_t out_cap) {
+                                         uint64_t idx_lo, uint64_t idx_hi) {
   const uint32_t q0 = P.q0, S = P.S, stride = P.stride;
   const uint32_t nw = W ? (uint32_t)W : P.W;
   const char* lb = reinterpret_cast<const char*>(lds);
@@ -168,8 +165,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
       }
       if (!live) bits = 0;
       if (bits)
-        es_exact_tail(P.cold, bits, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi, out,
-                      out_n, out_cap);
+        es_exact_tail(P.cold, bits, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
     } else if (W == 2) {
       const uint2 carry = *reinterpret_cast<const uint2*>(carry_tab + 2 * h);
       uint32_t b0 = 0xffffffffu, b1 = 0xffffffffu;
@@ -184,11 +180,9 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
       if (!live) b0 = b1 = 0;
       if (b0 | b1) {
         if (b0)
-          es_exact_tail(P.cold, b0, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi, out,
-                        out_n, out_cap);
+          es_exact_tail(P.cold, b0, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
         if (b1)
-          es_exact_tail(P.cold, b1, 1, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi, out,
-                        out_n, out_cap);
+          es_exact_tail(P.cold, b1, 1, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
       }
     } else {
       for (uint32_t w = 0; w < nw; ++w) {
@@ -206,97 +200,130 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
         }
         if (!live) bits = 0;
         if (bits)
-          es_exact_tail(P.cold, bits, w, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi, out,
-                        out_n, out_cap);
+          es_exact_tail(P.cold, bits, w, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
       }
     }
   }
+}
+
+// Strip u of a plan (u may be a replay of a valid strip for lanes past the range: `in` = false
+// makes its range empty).  Decodes the strip base over rings m+1.. in (h, l) form, sets up the
+// lane's mask-row addresses, then sweeps the strip.
+template <int F, int G, int W>
+__device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, uint64_t u,
+                                         bool in, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo,
+                                         uint64_t idx_hi) {
+  const uint64_t M0l = P.M0_lo, M0h = P.M0_hi;
+  const uint32_t q0 = P.q0, S = P.S;
+  const uint32_t row_bytes = 8u * (W ? (uint32_t)W : P.W);
+  cu32_t* mk = (cu32_t*)(P.masks);
+  uint32_t hu = 0;
+  uint64_t ll = 0, lh = 0;
+  uint64_t rem = u;
+  for (uint32_t j = 0; j < P.nb; ++j) {
+    const uint32_t r = P.base_radix[j];
+    uint32_t d;
+    if (j + 1 == P.nb) {
+      d = (uint32_t)rem;
+    } else {
+      const uint64_t qt = rem / r;
+      d = (uint32_t)(rem - qt * r);
+      rem = qt;
+    }
+    const uint32_t o = P.base_off[j] + d;
+    const uint64_t tl = P.term_l[2 * o], th = P.term_l[2 * o + 1];
+    const uint64_t nl = ll + tl;
+    lh = lh + th + (nl < ll ? 1ull : 0ull);
+    ll = nl;
+    uint32_t nh = hu + P.term_h[o];
+    if (es_ge128(lh, ll, M0h, M0l)) {
+      const uint64_t bl = ll < M0l ? 1ull : 0ull;
+      ll -= M0l;
+      lh = lh - M0h - bl;
+      nh += 1u;
+    }
+    if (nh >= q0) nh -= q0;
+    hu = nh;
+  }
+  const uint64_t zl = M0l - ll;  // z = M0 - l_u: l_u + L >= M0 <=> L >= z
+  const uint64_t zh = M0h - lh - (M0l < ll ? 1ull : 0ull);
+  // LDS byte addresses of this lane's mask rows for wrap clear (A0) / set (A1)
+  uint32_t A0[F + 1], A1[F + 1];
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    cu32_t* rec = mk + kMaskInfo * f;
+    const uint32_t m = rec[0];
+    const uint64_t bm = (uint64_t)hu * rec[2] + es_mod128(ll, lh, m, rec[3], rec[4], rec[5]) +
+                        rec[7];
+    const uint32_t x = (uint32_t)(bm % m);
+    uint32_t x1 = x + rec[6];
+    if (x1 >= m) x1 -= m;
+    A0[f] = rec[1] + row_bytes * x;
+    A1[f] = rec[1] + row_bytes * x1;
+  }
+  A0[F] = A1[F] = 0;
+  const uint64_t obase = u * S;
+  uint32_t sb = 0, se = 0;
+  if (in) {
+    sb = o_lo > obase ? (uint32_t)(o_lo - obase) : 0u;
+    se = o_hi - obase >= S ? S : (uint32_t)(o_hi - obase);
+  }
+  const bool full = in && sb == 0 && se == S;
+  if (__all(full))
+    es_sweep<F, G, W, false>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi);
+  else
+    es_sweep<F, G, W, true>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi);
 }
 
 template <int F, int G, int W>
 __global__ void __launch_bounds__(1024)
 es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, uint64_t o_hi,
-                uint64_t idx_lo, uint64_t idx_hi, unsigned long long* __restrict__ out,
-                unsigned long long* __restrict__ out_n, uint64_t out_cap) {
+                uint64_t idx_lo, uint64_t idx_hi) {
   extern __shared__ uint32_t lds[];
   for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
   __syncthreads();
-
-  const uint64_t M0l = P.M0_lo, M0h = P.M0_hi;
-  const uint32_t q0 = P.q0, S = P.S;
-  const uint32_t row_bytes = 8u * (W ? (uint32_t)W : P.W);
   const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  cu32_t* mk = (cu32_t*)(P.masks);
-
   // Wave-uniform strip loop: a lane past u_hi replays strip u_lo with an empty range.
   for (uint64_t base = u_lo + (uint64_t)blockIdx.x * blockDim.x; base < u_hi; base += step) {
     const uint64_t my = base + threadIdx.x;
     const bool in = my < u_hi;
-    const uint64_t u = in ? my : u_lo;
-    // strip base over rings m+1..w-1, summed in (h, l) form
-    uint32_t hu = 0;
-    uint64_t ll = 0, lh = 0;
-    uint64_t rem = u;
-    for (uint32_t j = 0; j < P.nb; ++j) {
-      const uint32_t r = P.base_radix[j];
-      uint32_t d;
-      if (j + 1 == P.nb) {
-        d = (uint32_t)rem;
-      } else {
-        const uint64_t qt = rem / r;
-        d = (uint32_t)(rem - qt * r);
-        rem = qt;
-      }
-      const uint32_t o = P.base_off[j] + d;
-      const uint64_t tl = P.term_l[2 * o], th = P.term_l[2 * o + 1];
-      const uint64_t nl = ll + tl;
-      lh = lh + th + (nl < ll ? 1ull : 0ull);
-      ll = nl;
-      uint32_t nh = hu + P.term_h[o];
-      if (es_ge128(lh, ll, M0h, M0l)) {
-        const uint64_t bl = ll < M0l ? 1ull : 0ull;
-        ll -= M0l;
-        lh = lh - M0h - bl;
-        nh += 1u;
-      }
-      if (nh >= q0) nh -= q0;
-      hu = nh;
-    }
-    const uint64_t zl = M0l - ll;  // z = M0 - l_u: l_u + L >= M0 <=> L >= z
-    const uint64_t zh = M0h - lh - (M0l < ll ? 1ull : 0ull);
-    // LDS byte addresses of this lane's mask rows for wrap clear (A0) / set (A1)
-    uint32_t A0[F + 1], A1[F + 1];
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      cu32_t* rec = mk + kMaskInfo * f;
-      const uint32_t m = rec[0];
-      const uint64_t bm = (uint64_t)hu * rec[2] + es_mod128(ll, lh, m, rec[3], rec[4], rec[5]) +
-                          rec[7];
-      const uint32_t x = (uint32_t)(bm % m);
-      uint32_t x1 = x + rec[6];
-      if (x1 >= m) x1 -= m;
-      A0[f] = rec[1] + row_bytes * x;
-      A1[f] = rec[1] + row_bytes * x1;
-    }
-    A0[F] = A1[F] = 0;
-    const uint64_t obase = u * S;
-    uint32_t sb = 0, se = 0;
-    if (in) {
-      sb = o_lo > obase ? (uint32_t)(o_lo - obase) : 0u;
-      se = o_hi - obase >= S ? S : (uint32_t)(o_hi - obase);
-    }
-    const bool full = in && sb == 0 && se == S;
-    if (__all(full))
-      es_sweep<F, G, W, false>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out,
-                               out_n, out_cap);
-    else
-      es_sweep<F, G, W, true>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi, out,
-                              out_n, out_cap);
+    es_strip<F, G, W>(P, lds, in ? my : u_lo, in, o_lo, o_hi, idx_lo, idx_hi);
   }
 }
 
-typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t,
-                         unsigned long long*, unsigned long long*, uint64_t);
+// Grid-mapped batch: one launch over all jobs of a shape; workgroup b serves strips
+// [first_u + (b - wg_start) * blockDim.x, ...) of the job whose workgroup range holds b (found by
+// binary search over the jobs' first workgroups).  The hardware dispatcher hands workgroups to
+// CUs as they free up, so uneven jobs balance at workgroup granularity, as in the single-plan
+// kernel.
+struct GridJob {
+  uint64_t wg_start;           // first workgroup of the job in the launch
+  uint64_t u_lo, u_hi, o_lo, o_hi, idx_lo, idx_hi;
+  uint32_t plan, pad;
+};
+
+template <int F, int G, int W>
+__global__ void __launch_bounds__(1024)
+es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
+                     uint32_t n_jobs) {
+  extern __shared__ uint32_t lds[];
+  const uint64_t b = blockIdx.x;
+  uint32_t lo = 0, hi = n_jobs;  // last job with wg_start <= b
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].wg_start <= b) lo = mid; else hi = mid;
+  }
+  const GridJob J = jobs[lo];
+  const HotPlan P = plans[J.plan];
+  for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
+  __syncthreads();
+  const uint64_t my = J.u_lo + (b - J.wg_start) * blockDim.x + threadIdx.x;
+  const bool in = my < J.u_hi;
+  es_strip<F, G, W>(P, lds, in ? my : J.u_lo, in, J.o_lo, J.o_hi, J.idx_lo, J.idx_hi);
+}
+
+typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
+typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
 
 // Instances: every F <= 12 with all masks in LDS for W = 1, 2 and runtime W; for W = 1 also
 // G = 2 or 4 of the masks in global memory.
@@ -312,24 +339,35 @@ KernelFn kernel_for(uint32_t W, uint32_t G) {
   return es_sieve_kernel<F, 0, 0>;
 }
 
-KernelFn pick_kernel(int F, uint32_t W, uint32_t G) {
-  switch (F) {
-    case 0: return kernel_for<0>(W, G);
-    case 1: return kernel_for<1>(W, G);
-    case 2: return kernel_for<2>(W, G);
-    case 3: return kernel_for<3>(W, G);
-    case 4: return kernel_for<4>(W, G);
-    case 5: return kernel_for<5>(W, G);
-    case 6: return kernel_for<6>(W, G);
-    case 7: return kernel_for<7>(W, G);
-    case 8: return kernel_for<8>(W, G);
-    case 9: return kernel_for<9>(W, G);
-    case 10: return kernel_for<10>(W, G);
-    case 11: return kernel_for<11>(W, G);
-    case 12: return kernel_for<12>(W, G);
-    default: return nullptr;
-  }
+template <int F>
+GridBatchFn grid_batch_for(uint32_t W, uint32_t G) {
+  if (G != 0) return nullptr;
+  if (W == 1) return es_grid_batch_kernel<F, 0, 1>;
+  if (W == 2) return es_grid_batch_kernel<F, 0, 2>;
+  return es_grid_batch_kernel<F, 0, 0>;
 }
+
+
+#define ES_F_SWITCH(FN, F, ...)        \
+  switch (F) {                         \
+    case 0: return FN<0>(__VA_ARGS__);   \
+    case 1: return FN<1>(__VA_ARGS__);   \
+    case 2: return FN<2>(__VA_ARGS__);   \
+    case 3: return FN<3>(__VA_ARGS__);   \
+    case 4: return FN<4>(__VA_ARGS__);   \
+    case 5: return FN<5>(__VA_ARGS__);   \
+    case 6: return FN<6>(__VA_ARGS__);   \
+    case 7: return FN<7>(__VA_ARGS__);   \
+    case 8: return FN<8>(__VA_ARGS__);   \
+    case 9: return FN<9>(__VA_ARGS__);   \
+    case 10: return FN<10>(__VA_ARGS__); \
+    case 11: return FN<11>(__VA_ARGS__); \
+    case 12: return FN<12>(__VA_ARGS__); \
+    default: return nullptr;           \
+  }
+
+KernelFn pick_kernel(int F, uint32_t W, uint32_t G) { ES_F_SWITCH(kernel_for, F, W, G) }
+GridBatchFn pick_grid_batch(int F, uint32_t W, uint32_t G) { ES_F_SWITCH(grid_batch_for, F, W, G) }
 
 struct Blob {
   std::vector<uint8_t> bytes;
@@ -361,6 +399,7 @@ struct es_engine {
   uint32_t* d_mask_info = nullptr;
   uint32_t* d_slow_info = nullptr;
   HotPlan hp{};
+  ColdPlan cold{};
   KernelFn kern = nullptr;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -394,9 +433,7 @@ int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, ui
     e->ev_used += 2;
     ES_HIP(hipEventRecord(a, e->stream));
   }
-  uint64_t cap = e->out_cap;
-  void* args[] = {&e->hp, &u_lo, &u_hi, &o_lo, &o_hi, &idx_lo, &idx_hi, &e->d_out, &e->d_count,
-                  &cap};
+  void* args[] = {&e->hp, &u_lo, &u_hi, &o_lo, &o_hi, &idx_lo, &idx_hi};
   ES_HIP(hipLaunchKernel((const void*)e->kern, dim3((uint32_t)grid), dim3(e->threads), args,
                          e->lds_bytes, e->stream));
   ES_HIP(hipGetLastError());
@@ -725,6 +762,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->max_grid = 1u << 30;  // one strip per thread; the dispatcher balances the tail
   ES_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   ES_HIP(hipMalloc(&e->dblob, bl.bytes.size()));
+  ES_HIP(hipMalloc(&e->d_out, std::max<uint64_t>(e->out_cap, 1) * 8));
   uint8_t* base = (uint8_t*)e->dblob;
   ColdPlan cold{};
   cold.M0_lo = (uint64_t)e->M0;
@@ -738,9 +776,12 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   cold.inner_ids = (const uint32_t*)(base + o_ids);
   cold.slow = (const uint32_t*)(base + o_slow);
   cold.slow_bits = (const uint32_t*)(base + o_sbits);
+  cold.out = e->d_out;
+  cold.out_n = (unsigned long long*)(base + o_cnt);
+  cold.out_cap = e->out_cap;
+  e->cold = cold;
   std::memcpy(bl.bytes.data() + o_cold, &cold, sizeof(cold));
   ES_HIP(hipMemcpy(e->dblob, bl.bytes.data(), bl.bytes.size(), hipMemcpyHostToDevice));
-  ES_HIP(hipMalloc(&e->d_out, std::max<uint64_t>(e->out_cap, 1) * 8));
   e->d_count = (unsigned long long*)(base + o_cnt);
   e->d_mask_info = (uint32_t*)(base + o_mask);
   e->d_slow_info = (uint32_t*)(base + o_slow);
@@ -941,6 +982,192 @@ int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t
   e->ev_used = 0;
   e->launches = 0;
   e->positions = 0;
+  return ES_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------ batches
+
+struct es_batch {
+  struct Job {
+    es_engine* e = nullptr;
+    uint64_t t = 0, idx_lo = 0, idx_hi = 0, cap = 0;
+    void* dmem = nullptr;                   // mask records, slow records, cold plan, counter
+    unsigned long long* d_out = nullptr;
+    unsigned long long* d_count = nullptr;
+    HotPlan hp{};
+  };
+  hipStream_t stream = nullptr;
+  uint64_t out_cap = 1u << 16;
+  std::vector<Job> jobs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" {
+
+int es_batch_create(uint64_t out_cap_per_job, es_batch** out) {
+  if (!out) return fail(ES_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::unique_ptr<es_batch> b(new es_batch());
+  b->out_cap = out_cap_per_job ? out_cap_per_job : (1u << 16);
+  ES_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  ES_HIP(hipEventCreate(&b->ev0));
+  ES_HIP(hipEventCreate(&b->ev1));
+  *out = b.release();
+  return ES_OK;
+}
+
+int es_batch_destroy(es_batch* b) {
+  if (!b) return ES_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto& j : b->jobs) {
+    hipFree(j.dmem);
+    hipFree(j.d_out);
+  }
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return ES_OK;
+}
+
+int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_t idx_hi,
+                 uint32_t* job_id) {
+  if (!b || !e) return fail(ES_ERR_INVALID, "null argument");
+  if (idx_hi > (uint64_t)e->R || idx_lo >= idx_hi) return fail(ES_ERR_INVALID, "window outside [0, R)");
+  if (e->G != 0) return fail(ES_ERR_LIMIT, "batched launches read every mask from LDS (G = 0)");
+  es_batch::Job j;
+  j.e = e;
+  j.t = t;
+  j.idx_lo = idx_lo;
+  j.idx_hi = idx_hi;
+  j.cap = b->out_cap;
+  // block constants of this job
+  std::vector<uint32_t> tc(std::max<uint32_t>(e->n_filters, 1), 0);
+  for (uint32_t f = 0; f < e->n_filters; ++f) {
+    const uint32_t q = e->filt_q[f];
+    tc[f] = (uint32_t)(((uint64_t)(t % q) * mod_u128(e->M, q)) % q);
+  }
+  std::vector<uint32_t> mi(e->mask_info), si(e->slow_info);
+  for (uint32_t f = 0; f < e->F; ++f) mi[kMaskInfo * f + 7] = tc[e->mask_src[f]] % e->mask_mod[f];
+  for (uint32_t s = 0; s < e->n_slow; ++s) si[kSlowInfo * s + 5] = tc[e->slow_order[s]];
+  Blob bl;
+  const size_t o_mi = bl.put(mi.data(), mi.size() * 4);
+  const size_t o_si = bl.put(si.data(), si.size() * 4);
+  const size_t o_cold = bl.put(nullptr, sizeof(ColdPlan));
+  const size_t o_cnt = bl.put(nullptr, 8);
+  ES_HIP(hipMalloc(&j.dmem, bl.bytes.size()));
+  ES_HIP(hipMalloc((void**)&j.d_out, j.cap * 8));
+  uint8_t* base = (uint8_t*)j.dmem;
+  ColdPlan cold = e->cold;
+  cold.slow = (const uint32_t*)(base + o_si);
+  cold.out = j.d_out;
+  cold.out_n = (unsigned long long*)(base + o_cnt);
+  cold.out_cap = j.cap;
+  std::memcpy(bl.bytes.data() + o_cold, &cold, sizeof(cold));
+  ES_HIP(hipMemcpy(j.dmem, bl.bytes.data(), bl.bytes.size(), hipMemcpyHostToDevice));
+  j.d_count = (unsigned long long*)(base + o_cnt);
+  j.hp = e->hp;
+  j.hp.masks = (const uint32_t*)(base + o_mi);
+  j.hp.cold = (const ColdPlan*)(base + o_cold);
+  if (job_id) *job_id = (uint32_t)b->jobs.size();
+  b->jobs.push_back(j);
+  return ES_OK;
+}
+
+int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
+  if (!b) return fail(ES_ERR_INVALID, "null argument");
+  for (auto& j : b->jobs) ES_HIP(hipMemsetAsync(j.d_count, 0, 8, b->stream));
+  // Group jobs by kernel instance, workgroup size and occupancy class: a launch allocates its
+  // largest plan's LDS for every workgroup, so only plans that fit as many workgroups per CU
+  // (2048 lanes and 160 KiB of LDS per CU) share one; a small plan is never held to a large
+  // plan's occupancy.
+  std::vector<std::vector<uint32_t>> groups;
+  struct Key { GridBatchFn fn; uint32_t threads, occ; };
+  std::vector<Key> keys;
+  for (uint32_t i = 0; i < b->jobs.size(); ++i) {
+    const es_engine* e = b->jobs[i].e;
+    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0);
+    if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
+    const uint32_t occ = std::min<uint32_t>(2048u / e->threads,
+                                            163840u / std::max<uint32_t>(e->lds_bytes, 1));
+    size_t g = 0;
+    while (g < keys.size() &&
+           !(keys[g].fn == fn && keys[g].threads == e->threads && keys[g].occ == occ))
+      ++g;
+    if (g == keys.size()) {
+      keys.push_back({fn, e->threads, occ});
+      groups.emplace_back();
+    }
+    groups[g].push_back(i);
+  }
+  std::vector<void*> scratch;
+  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  for (size_t g = 0; g < groups.size(); ++g) {
+    // every job's strips as consecutive workgroups of one launch
+    const uint32_t threads = keys[g].threads;
+    std::vector<HotPlan> plans;
+    std::vector<GridJob> gj;
+    uint32_t lds_max = 4;
+    uint64_t wgs = 0;
+    for (uint32_t ji : groups[g]) {
+      const es_batch::Job& j = b->jobs[ji];
+      const es_engine* e = j.e;
+      GridJob J{};
+      J.plan = (uint32_t)plans.size();
+      plans.push_back(j.hp);
+      lds_max = std::max(lds_max, e->lds_bytes);
+      J.o_lo = j.idx_lo / e->r0;
+      J.o_hi = (j.idx_hi + e->r0 - 1) / e->r0;
+      J.u_lo = J.o_lo / e->S;
+      J.u_hi = (J.o_hi + e->S - 1) / e->S;
+      J.idx_lo = j.idx_lo;
+      J.idx_hi = j.idx_hi;
+      J.wg_start = wgs;
+      wgs += (J.u_hi - J.u_lo + threads - 1) / threads;
+      gj.push_back(J);
+    }
+    if (wgs > 0x7fffffffull) return fail(ES_ERR_LIMIT, "batch too large for one grid");
+    void *d_plans = nullptr, *d_jobs = nullptr;
+    ES_HIP(hipMalloc(&d_plans, plans.size() * sizeof(HotPlan)));
+    ES_HIP(hipMalloc(&d_jobs, gj.size() * sizeof(GridJob)));
+    scratch.push_back(d_plans);
+    scratch.push_back(d_jobs);
+    ES_HIP(hipMemcpyAsync(d_plans, plans.data(), plans.size() * sizeof(HotPlan),
+                          hipMemcpyHostToDevice, b->stream));
+    ES_HIP(hipMemcpyAsync(d_jobs, gj.data(), gj.size() * sizeof(GridJob), hipMemcpyHostToDevice,
+                          b->stream));
+    const GridBatchFn fn = keys[g].fn;
+    if (lds_max > 65536)
+      ES_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_max));
+    const HotPlan* pp = (const HotPlan*)d_plans;
+    const GridJob* jp = (const GridJob*)d_jobs;
+    uint32_t nj = (uint32_t)gj.size();
+    void* args[] = {&pp, &jp, &nj};
+    ES_HIP(hipLaunchKernel((const void*)fn, dim3((uint32_t)std::max<uint64_t>(wgs, 1)),
+                           dim3(threads), args, lds_max, b->stream));
+    ES_HIP(hipGetLastError());
+  }
+  ES_HIP(hipEventRecord(b->ev1, b->stream));
+  ES_HIP(hipStreamSynchronize(b->stream));
+  for (void* p : scratch) ES_HIP(hipFree(p));
+  float t = 0.f;
+  ES_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+  if (ms) *ms = t;
+  if (launches) *launches = (uint32_t)groups.size();
+  return ES_OK;
+}
+
+int es_batch_collect(es_batch* b, uint32_t job, uint64_t* ids, uint64_t cap, uint64_t* n) {
+  if (!b || !n || job >= b->jobs.size()) return fail(ES_ERR_INVALID, "bad job");
+  const es_batch::Job& j = b->jobs[job];
+  unsigned long long c = 0;
+  ES_HIP(hipMemcpy(&c, j.d_count, 8, hipMemcpyDeviceToHost));
+  *n = c;
+  if (c > j.cap) return fail(ES_ERR_OVERFLOW, "survivor buffer overflow");
+  if (ids && c) ES_HIP(hipMemcpy(ids, j.d_out, std::min<uint64_t>(c, cap) * 8, hipMemcpyDeviceToHost));
   return ES_OK;
 }
 

@@ -72,6 +72,11 @@ def lib() -> C.CDLL:
         "es_engine_collect": (C.c_int, [eng, u64p, C.c_uint64, u64p, u64p, C.POINTER(C.c_int)]),
         "es_engine_sync": (C.c_int, [eng]),
         "es_engine_kernel_time": (C.c_int, [eng, C.POINTER(C.c_double), u64p, u64p]),
+        "es_batch_create": (C.c_int, [C.c_uint64, C.POINTER(eng)]),
+        "es_batch_destroy": (C.c_int, [eng]),
+        "es_batch_add": (C.c_int, [eng, eng, C.c_uint64, C.c_uint64, C.c_uint64, u32p]),
+        "es_batch_run": (C.c_int, [eng, C.POINTER(C.c_double), u32p]),
+        "es_batch_collect": (C.c_int, [eng, C.c_uint32, u64p, C.c_uint64, u64p]),
         "es_fpass_create": (C.c_int, [C.c_uint32, u32p, u32p, u64p, C.c_uint32, C.c_uint32,
                                       C.c_uint64, C.POINTER(eng)]),
         "es_fpass_destroy": (C.c_int, [eng]),

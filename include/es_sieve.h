@@ -129,6 +129,26 @@ int es_engine_sync(es_engine* e);
  * stream), their launch count, and the positions they covered. */
 int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t* positions);
 
+/* ---- Batched launches over many plans and blocks ---------------------------------------
+ * A batch holds jobs (engine, block t, window [idx_lo, idx_hi)); es_batch_run sieves all of
+ * them with one launch per kernel shape (mask count, words per ring-0 row, workgroup size):
+ * the jobs' strips are laid out as consecutive workgroup ranges of one grid, each workgroup
+ * finds its job by binary search over the ranges and loads that plan's LDS tables, and the
+ * hardware dispatcher balances uneven jobs at workgroup granularity.  Each job has its own block
+ * constants and survivor buffer, so its survivors equal those of es_engine_scan_window on the
+ * same engine.  No reference counterpart: the reference runs one plan per process
+ * (harness/grind.py) and one launch per chunk (sieve/gpu_sieve_v9.py:500-506). */
+typedef struct es_batch es_batch;
+
+int es_batch_create(uint64_t out_cap_per_job, es_batch** out);
+int es_batch_destroy(es_batch* b);
+/* The engine must outlive the batch. */
+int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_t idx_hi,
+                 uint32_t* job_id);
+/* Synchronous; *ms = device time of all launches, *launches = launches issued. */
+int es_batch_run(es_batch* b, double* ms, uint32_t* launches);
+int es_batch_collect(es_batch* b, uint32_t job, uint64_t* ids, uint64_t cap, uint64_t* n);
+
 /* ---- Candidate filter pass over a 128-bit value stream (HBM-bound) ------------------------
  * Reference interface replaced: the late scalar filters the word-mask kernel runs on the
  * candidates its masks leave, emitting original IDs (sieve/es_word_masks.py:87-92 with the

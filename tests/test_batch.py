@@ -1,0 +1,69 @@
+"""Batched launches (k = 300..320 shapes): every job's survivors equal the
+unbatched engine's on the same block and window."""
+from __future__ import annotations
+
+import random
+
+import pytest
+
+import es_plan as EP
+from conftest import bfile
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(k, **tune):
+    import gpu_sieve_mi355x as G
+    return G.GpuSieve(EP.Plan(k, dict(tune)))
+
+
+def test_batch_equals_unbatched_dense():
+    """Filter-prefix engines (dense survivors) of mixed shapes: W = 1 and 2, several F."""
+    import es_batch
+    rng = random.Random(2026)
+    b = es_batch.Batch(out_cap_per_job=1 << 20)
+    jobs = []
+    for k in (300, 301, 308, 317):
+        eng = _engine(k, mi_filter_limit=4, max_survivors=1 << 20)
+        for _ in range(2):
+            t = rng.randrange(1 << 20)
+            lo = rng.randrange(eng.plan.R - (1 << 22))
+            hi = lo + rng.randrange(1 << 21, 1 << 22)
+            jobs.append((b.add(eng, t, lo, hi), eng, t, lo, hi))
+    print(f"case batch dense jobs={len(jobs)}", flush=True)
+    ms, launches = b.run()
+    assert launches >= 2
+    total = 0
+    for jid, eng, t, lo, hi in jobs:
+        want, _ = eng.scan_window(t, lo, hi)
+        got = b.collect(jid)
+        assert got == sorted(want) and len(got) > 0
+        total += len(got)
+    print(f"  survivors={total} ms={ms:.2f}", flush=True)
+
+
+def test_batch_full_filters_finds_g():
+    """k = 300..320, default planner, full filters: a window around g(k) in its block returns
+    g(k), and seeded windows agree with the unbatched engine."""
+    import es_batch
+    bf = bfile()
+    rng = random.Random(7)
+    b = es_batch.Batch()
+    jobs = []
+    for k in range(300, 321):
+        eng = _engine(k)
+        idx, t = eng.plan.n_to_idx(bf[k])
+        lo = max(0, idx - (1 << 26))
+        hi = min(eng.plan.R, lo + (1 << 27))
+        jobs.append((b.add(eng, t, lo, hi), eng, t, lo, hi, bf[k]))
+        t2 = rng.randrange(1 << 20)
+        lo2 = rng.randrange(eng.plan.R - (1 << 27))
+        jobs.append((b.add(eng, t2, lo2, lo2 + (1 << 27)), eng, t2, lo2, lo2 + (1 << 27), None))
+    print(f"case batch full jobs={len(jobs)}", flush=True)
+    b.run()
+    for jid, eng, t, lo, hi, g in jobs:
+        got = b.collect(jid)
+        want, mn = eng.scan_window(t, lo, hi)
+        assert got == sorted(want)
+        if g is not None:
+            assert eng.plan.n_to_idx(g)[0] in got and mn == g

@@ -41,10 +41,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true", help="batch: autotune each k first")
-    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch"), default="sieve",
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch, k200: autotune the launch shape of each wheel first")
+    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch", "k200"),
+                    default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
-                         "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters)")
+                         "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters); "
+                         "batch: k=300..320 batched launches; k200: full-power wheel window")
     ap.add_argument("--fpass-log2", type=int, default=30)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -57,6 +60,8 @@ def main() -> int:
         return bench_fpass(a)
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "k200":
+        return bench_k200(a)
     if a.workload == "copy":
         import torch
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")       # 1 GiB
@@ -274,6 +279,78 @@ def bench_batch(a) -> int:
                    "tuned": {str(k): v for k, v in tuned.items()}},
         "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
         "survivors_total": sum(len(x) for x in single)}), flush=True)
+    return 0
+
+
+def bench_k200(a) -> int:
+    """k = 200 on its full-power wheel (p <= 23 full powers, extended to M <= 2^100; 37 filter
+    primes): one seeded window of 2^36 positions at a seeded block t in [0, 2^20), scanned
+    `steps` times; the CPU oracle is timed on the window's first 2^28 positions and must agree
+    there; the full window's survivor hash is reported."""
+    import random
+
+    import numpy as np
+
+    import es_plan
+    import gpu_sieve_mi355x as G
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import es_oracle as O
+    k, N = 200, 1 << 36
+    wheel = es_plan.fullpower_wheel(k)
+    tune = {"max_survivors": 1 << 20}
+    if a.autotune:
+        import es_autotune
+        best, _ = es_autotune.autotune(k, wheel, probe_log2=36, method="coord")
+        tune.update(best)
+    plan = es_plan.Plan(k, tune, wheel)
+    eng = G.GpuSieve(plan)
+    rng = random.Random(200)
+    t = rng.randrange(1 << 20)
+    lo = rng.randrange(plan.R - N)
+    for _ in range(max(1, a.warmup)):
+        eng.scan_window(t, lo, lo + N)
+    eng.sync()
+    eng.kernel_time()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ids, mn = eng.scan_window(t, lo, lo + N)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    dev_ms, launches, dev_pos = eng.kernel_time()
+    ns = sorted(plan.idx_to_n(int(i), t) for i in ids)
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        sub = 1 << 28
+        c0 = time.perf_counter()
+        cnt, want = O.c_scan(k, plan.wheel, plan.rest, t, lo, lo + sub, threads=threads)
+        cdt = time.perf_counter() - c0
+        got = np.sort(np.asarray([i for i in ids if lo <= int(i) < lo + sub], dtype=np.uint64))
+        assert cnt == len(want) and np.array_equal(got, want)
+        cpu = {"value": sub / cdt, "unit": "positions/s", "cores": threads, "kind": "port",
+               "sample": f"first 2^28 positions of the window ({cdt:.1f} s), survivors agree"}
+    kern_rate = dev_pos / (dev_ms / 1e3)
+    ops_ref = 12 + 10 * plan.expected_filter_evals()
+    print(json.dumps({
+        "metric": "residue positions sieved/s (k=200 full-power wheel)", "value": a.steps * N / wall,
+        "unit": "positions/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u128/u32 integer", "data": "synthetic seeded window",
+        "config": {"workload": "k=200 full-power wheel, 2^36 positions at seeded block",
+                   "t": t, "idx_lo": lo, "wheel": [list(x) for x in plan.wheel],
+                   "R": plan.R, "filters": len(plan.rest), "autotuned": bool(a.autotune),
+                   "tune": {x: plan.tun[x] for x in ("mi_masks", "threads", "mi_strip_target",
+                                                     "mi_inner_radix")},
+                   "kernel_ring0_radix": eng._d.r0, "strip_len": eng.strip_len},
+        "survivors": len(ns), "survivor_hash": O.survivor_hash(ns),
+        "min": str(mn) if mn is not None else None,
+        "roofline": {"bound": "valu-int32", "achieved": ops_ref * kern_rate / 1e12,
+                     "peak": INT32_PEAK / 1e12, "unit": "TOP/s",
+                     "frac": ops_ref * kern_rate / INT32_PEAK, "traffic": None,
+                     "ops_per_position_reference": round(ops_ref, 3),
+                     "kernel_avg_launch_ms": dev_ms / max(1, launches),
+                     "kernel_positions_per_s": kern_rate},
+        "cpu_baseline": cpu}), flush=True)
     return 0
 
 

@@ -185,8 +185,11 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
           es_exact_tail(P.cold, b1, 1, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
       }
     } else {
+      // carrying slots are the suffix [j, r0) of the sorted ring-0 slots
+      const uint32_t j = reinterpret_cast<const uint16_t*>(carry_tab)[h];
       for (uint32_t w = 0; w < nw; ++w) {
-        const uint32_t carry = carry_tab[h * nw + w];
+        const int d = (int)j - 32 * (int)w;
+        const uint32_t carry = d <= 0 ? 0xffffffffu : (d >= 32 ? 0u : 0xffffffffu << d);
         uint32_t bits = 0xffffffffu;
         if (F == 0) {
           const uint32_t n = P.r0 - 32u * w;
@@ -517,7 +520,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->w = w;
   e->M = M;
   e->R = R;
-  e->q0 = d->ring_mod[0];
+  e->q0 = d->inner_mod ? d->inner_mod : d->ring_mod[0];
+  if (e->q0 % d->ring_mod[0] != 0 || M % e->q0 != 0 || e->q0 >= (1u << 31))
+    return fail(ES_ERR_INVALID, "inner_mod must divide M, be a multiple of ring_mod[0] and < 2^31");
   e->r0 = d->ring_radix[0];
   e->M0 = M / e->q0;
   e->W = (e->r0 + 31) / 32;
@@ -559,10 +564,18 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   const uint32_t target = d->strip_target ? d->strip_target : 1024;
   uint32_t m = 0;
   uint64_t S = 1;
+  // strip rings 1..m: the product of leading radices nearest the target in log scale (the
+  // ring that crosses the target is taken when S * nxt < target^2)
   while (1 + m < w) {
     const uint64_t nxt = S * d->ring_radix[1 + m];
-    if (m > 0 && nxt > target) break;
     if (nxt > (1u << 22)) break;
+    if (m > 0 && nxt > target) {
+      if (nxt * S < (uint64_t)target * target) {
+        S = nxt;
+        ++m;
+      }
+      break;
+    }
     S = nxt;
     ++m;
   }
@@ -574,7 +587,12 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // ---- mask tables: rows [0, 2m) of 2W words (c0, c1 per ring-0 word).  Staged masks go to
   // LDS, except that for W = 1 the G smallest tables are read from global memory through L1.
   const uint32_t budget_words = (d->lds_budget ? d->lds_budget : 65536u) / 4;
-  const uint32_t carry_words = q0 * W;
+  // Carry layout.  W <= 2: one row of W words per h (bit i set iff h + e_i >= q0).  W > 2:
+  // slots are sorted by e_i, so the carrying slots of row h are a suffix [j(h), r0); the table
+  // holds j(h) as u16 (q0 entries), and the kernel builds each carry word from it.
+  const bool carry_suffix = W > 2;
+  if (carry_suffix && r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
+  const uint32_t carry_words = carry_suffix ? (q0 + 1) / 2 : q0 * W;
   std::vector<uint32_t> staged;  // indices into the descriptor's mask list
   {
     size_t used = carry_words + 8;
@@ -636,9 +654,17 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->mask_src = msrc;
   const uint32_t carry_off = (uint32_t)((lds.size() + 3) & ~size_t(3));
   lds.resize(carry_off + carry_words, 0);
-  for (uint32_t h = 0; h < q0; ++h)
-    for (uint32_t i = 0; i < r0; ++i)
-      if (h + e_sorted[i] >= q0) lds[carry_off + (size_t)h * W + (i >> 5)] |= 1u << (i & 31);
+  if (carry_suffix) {
+    for (uint32_t h = 0; h < q0; ++h) {
+      const uint32_t j = (uint32_t)(std::lower_bound(e_sorted.begin(), e_sorted.begin() + r0,
+                                                     q0 - h) - e_sorted.begin());
+      lds[carry_off + h / 2] |= j << (16 * (h & 1));
+    }
+  } else {
+    for (uint32_t h = 0; h < q0; ++h)
+      for (uint32_t i = 0; i < r0; ++i)
+        if (h + e_sorted[i] >= q0) lds[carry_off + (size_t)h * W + (i >> 5)] |= 1u << (i & 31);
+  }
   if (lds.size() > 40960) return fail(ES_ERR_LIMIT, "ring-0 tables exceed 160 KiB of LDS");
   e->lds_bytes = (uint32_t)(lds.size() * 4);
 

@@ -9,7 +9,7 @@ in a JSON cache keyed by (k, wheel, device).
 Two search methods: "grid" times every point of the space; "coord" (coordinate descent) starts
 from the default setting and sweeps one dimension at a time with the others held, keeping a
 change only when it beats the current best by more than `margin`, until a full sweep changes
-nothing -- 13 probes per sweep over the default space instead of 96.
+nothing -- 14 probes per sweep over the default space instead of 120.
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ import es_plan
 import gpu_sieve_mi355x as G
 
 SPACE = {
-    "mi_strip_target": [64, 256, 1024, 4096],
+    "mi_strip_target": [64, 256, 1024, 4096, 16384],
     "threads": [64, 128, 256, 512],
     "mi_masks": [6, 8, 10],
     "mi_global_masks": [0, 2],

@@ -29,7 +29,7 @@ class PlanDesc(C.Structure):
         ("mask_bits", u32p), ("n_mask_words", C.c_uint32),
         ("strip_target", C.c_uint32), ("threads", C.c_uint32), ("out_cap", C.c_uint64),
         ("lds_budget", C.c_uint32), ("global_masks", C.c_uint32), ("launch_log2", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("inner_mod", C.c_uint32),
     ]
 
 

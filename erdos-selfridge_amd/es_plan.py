@@ -47,6 +47,7 @@ TUNABLES = {
     "mi_masks": 8,             # word-mask prefilters staged in LDS
     "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
     "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
+    "mi_inner_radix": 128,     # ring 0 mod p^T wider than this is swept as digit rings (0: never)
 }
 
 
@@ -206,6 +207,61 @@ def plan_wheel(k: int, ps: list[int], log2E: float, tun: dict) -> list[tuple[int
                      or prod(partial_count(k, p, T) for p, T in wheel) >= (1 << 62)):
         drop_weakest_digit(wheel)
     return wheel
+
+
+def fullpower_wheel(k: int, p_full: int = 23, m_cap_bits: float = 100.0,
+                    r_cap_bits: int = 62) -> list[tuple[int, int]]:
+    """Wheel of full prime powers q_p for every p <= p_full, extended by the exponent choice
+    (p, T) for larger primes that maximises M/R (integers covered per position) subject to
+    M <= 2^m_cap_bits and R < 2^r_cap_bits.  The search is exhaustive: an item can only enter
+    when its radix fits the R budget the full powers leave, which keeps the candidate set small
+    (at k = 200: 29, 41, 53, 67, 101, 103 with T = 1).  Ties go to the smaller R, then to the
+    lexicographically smaller item list."""
+    ps = primes_upto(k)
+    base = [(p, len(base_digits(k, p))) for p in ps if p <= p_full]
+    logM = sum(T * math.log2(p) for p, T in base)
+    R0 = prod(partial_count(k, p, T) for p, T in base)
+    M0 = prod(p ** T for p, T in base)
+    assert logM <= m_cap_bits and R0 < (1 << r_cap_bits), "full powers exceed the caps"
+    r_room = ((1 << r_cap_bits) - 1) // R0          # R0 * extra < 2^r_cap_bits
+    m_room = (1 << int(math.floor(m_cap_bits))) // M0 if m_cap_bits == int(m_cap_bits) else None
+    items = []                                      # (p, T, q, count)
+    for p in ps:
+        if p <= p_full:
+            continue
+        for T in range(1, len(base_digits(k, p)) + 1):
+            c = partial_count(k, p, T)
+            if c <= r_room and math.log2(p) * T <= m_cap_bits - logM:
+                items.append((p, T, p ** T, c))
+    best = (0.0, 1, ())                             # (log2 M/R gain, R factor, items)
+    budget = [1 << 20]
+
+    def fits_m(mq):
+        if m_room is not None:
+            return mq <= m_room
+        return math.log2(M0 * mq) <= m_cap_bits
+
+    def dfs(i, mq, rc, gain, chosen, used):
+        nonlocal best
+        budget[0] -= 1
+        assert budget[0] > 0, "wheel search too large"
+        key = (gain, -rc, tuple(chosen))
+        if (key[0] > best[0] + 1e-12 or (abs(key[0] - best[0]) <= 1e-12 and
+                                         (rc < best[1] or (rc == best[1] and
+                                                           tuple(chosen) < best[2])))):
+            best = (gain, rc, tuple(chosen))
+        for j in range(i, len(items)):
+            p, T, q, c = items[j]
+            if p in used or rc * c > r_room or not fits_m(mq * q):
+                continue
+            used.add(p)
+            chosen.append((p, T))
+            dfs(j + 1, mq * q, rc * c, gain + math.log2(q / c), chosen, used)
+            chosen.pop()
+            used.discard(p)
+
+    dfs(0, 1, 1, 0.0, [], set())
+    return sorted(base + list(best[2]))
 
 
 def order_filters(k: int, filt: list[int], info: dict, wheelT: dict[int, int]) -> list[int]:

@@ -44,8 +44,46 @@ def pack_bits(q: int, residues) -> np.ndarray:
     return words
 
 
+def kernel_rings(plan: Plan, max_inner: int = 64):
+    """Rings as the kernel sweeps them: the plan's rings, except that a ring 0 mod p^T (T >= 2)
+    whose radix exceeds max_inner is passed as digit rings -- its lowest digits grouped while
+    their radix stays <= max_inner (at least one digit), then one ring per further digit.  The
+    ascending admissible residues of p^T are the mixed radix over the digits (digit 0 fastest,
+    digit j over [a_j, p)), so the idx order is unchanged.  Returns (moduli, radices, terms,
+    inner_mod, f) with inner_mod = p^T (0 when ring 0 is not split) and f = plan ring-0 radix
+    divided by the kernel's (outer indices of the plan = f outer indices of the kernel)."""
+    if not plan.rings:
+        return [1], [1], [[0]], 0, 1
+    mods = [q for q, _ in plan.rings]
+    radix = list(plan.radix)
+    terms = plan.ring_terms()
+    p, T = plan.wheel[0]
+    if max_inner <= 0 or radix[0] <= max_inner or T < 2:
+        return mods, radix, terms, 0, 1
+    a = base_digits(plan.k, p)[:T]
+    groups = [[0]]
+    rad = p - a[0]
+    for j in range(1, T):
+        if len(groups) == 1 and rad * (p - a[j]) <= max_inner:
+            groups[0].append(j)
+            rad *= p - a[j]
+        else:
+            groups.append([j])
+    M, b0 = plan.M, plan.basis[0]
+    g_mods, g_radix, g_terms = [], [], []
+    for g in groups:
+        rhos = [0]
+        for j in g:                      # lowest digit of the group fastest
+            rhos = [r + d * p ** j for d in range(a[j], p) for r in rhos]
+        g_mods.append(p ** len(g))
+        g_radix.append(len(rhos))
+        g_terms.append([(r * b0) % M for r in rhos])
+    f = radix[0] // g_radix[0]
+    return g_mods + mods[1:], g_radix + radix[1:], g_terms + terms[1:], p ** T, f
+
+
 def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
-                 lds_budget: int = 65536):
+                 lds_budget: int = 65536, r0: int | None = None, q0: int | None = None):
     """Word-mask prefilters: low-digit conditions mod p^D (T_p < D <= t_p, p^D <= MASK_MOD_CAP)
     of the filter primes -- each a necessary condition of its full test.  Candidates are taken
     by least conditional pass rate, one depth per prime, while their LDS rows (each table held
@@ -53,8 +91,10 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     Returns [(filter_index, D, modulus, conditional pass rate)]."""
     k = plan.k
     rest = plan.rest if limit is None else plan.rest[:limit]
-    r0 = plan.radix[0] if plan.radix else 1
-    q0 = plan.rings[0][0] if plan.rings else 1
+    if r0 is None:
+        r0 = plan.radix[0] if plan.radix else 1
+    if q0 is None:
+        q0 = plan.rings[0][0] if plan.rings else 1
     words = (r0 + 31) // 32
     cands = []
     for fi, p in enumerate(rest):
@@ -66,7 +106,9 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
             rate = (partial_count(k, p, d) / p ** d) / base
             cands.append((rate, fi, d, p ** d))
     cands.sort(key=lambda c: (c[0], c[1], -c[2]))
-    left = lds_budget // 4 - q0 * words - 8
+    # carry table: q0 rows of `words` words, or q0 u16 suffix starts when words > 2
+    carry = (q0 + 1) // 2 if words > 2 else q0 * words
+    left = lds_budget // 4 - carry - 8
     taken, out = set(), []
     for rate, fi, d, m in cands:
         if len(out) >= n_masks:
@@ -105,12 +147,11 @@ class _Desc:
     def __init__(self, plan: Plan, tun: dict):
         k = plan.k
         M = plan.M
-        if plan.rings:
-            mods = [q for q, _ in plan.rings]
-            radix = list(plan.radix)
-            terms = [t for ring in plan.ring_terms() for t in ring]
-        else:
-            mods, radix, terms = [1], [1], [0]
+        mods, radix, rterms, self.inner_mod, self.f = kernel_rings(
+            plan, int(tun.get("mi_inner_radix", 128)))
+        terms = [t for ring in rterms for t in ring]
+        self.r0 = radix[0]
+        self.q0 = self.inner_mod or mods[0]
         self.ring_mod = np.asarray(mods, dtype=np.uint32)
         self.ring_radix = np.asarray(radix, dtype=np.uint32)
         self.term_lo = np.asarray([t & ((1 << 64) - 1) for t in terms], dtype=np.uint64)
@@ -132,7 +173,8 @@ class _Desc:
         self.filt_off = np.asarray(offs or [0], dtype=np.uint32)
         self.filt_bits = np.concatenate(words) if words else np.zeros(1, dtype=np.uint32)
         n_masks = min(MAX_MASKS, int(tun.get("mi_masks", 8)))
-        self.masks = choose_masks(plan, n_masks, limit, int(tun.get("mi_lds_budget", 65536)))
+        self.masks = choose_masks(plan, n_masks, limit, int(tun.get("mi_lds_budget", 65536)),
+                                  self.r0, self.q0)
         self.slow = slow_order(plan, self.masks, limit)
         mm, ms, mo, mw = [], [], [], []
         nw = 0
@@ -179,6 +221,7 @@ class _Desc:
         d.lds_budget = int(tun.get("mi_lds_budget", 65536))
         d.global_masks = int(tun.get("mi_global_masks", 0))
         d.launch_log2 = int(tun.get("mi_launch_log2", 37))
+        d.inner_mod = self.inner_mod
         self.desc = d
 
 
@@ -267,7 +310,9 @@ class GpuSieve:
         self.threads = int(P.tun["threads"])
         self.chunk = 1 << int(P.tun.get("mi_launch_log2", 37))
         self.R_outer = P.R // self.r0
-        assert self.R_outer == info.R_outer and self.R_outer * self.r0 == P.R
+        # the kernel may sweep ring 0 as digit rings: one plan outer index = f kernel ones
+        self._f = self._d.f
+        assert self.R_outer * self._f == info.R_outer and self.R_outer * self.r0 == P.R
         self.ochunk = max(1, self.chunk // self.r0)
         self.strip_len = info.strip_len
         self.mask_mode = f"mi355x-mask{info.n_masks_used}-l1{info.global_masks}"
@@ -287,7 +332,7 @@ class GpuSieve:
     # -- reference contract ------------------------------------------------------------
     def _launch(self, ostart: int, ocount: int) -> None:
         """ostart/ocount are OUTER indices; each covers r0 positions.  Asynchronous."""
-        nat.check(nat.lib().es_engine_scan(self._h, int(ostart), int(ocount)))
+        nat.check(nat.lib().es_engine_scan(self._h, int(ostart) * self._f, int(ocount) * self._f))
         self.launches += 1
 
     def set_block(self, t: int) -> None:

@@ -42,7 +42,15 @@ typedef struct es_engine es_engine;
 
 /* Plan descriptor.  Rings are in the reference's ring order (Plan.wheel after the
  * (-radix, p) sort, sieve/gpu_sieve_v9.py:366); ring 0 varies fastest in idx.  An empty
- * wheel is passed as one ring with modulus 1, radix 1, residue 0, term 0. */
+ * wheel is passed as one ring with modulus 1, radix 1, residue 0, term 0.
+ *
+ * A ring mod p^T may be passed as digit rings instead: its ascending admissible residues are
+ * a mixed radix over the base-p digits (digit 0 fastest, digit j ranging over [a_j, p)), so
+ * consecutive groups of digits, lowest first, are rings of their own with the same idx order.
+ * Each group is passed with modulus p^(digits in the group) and its terms (rho * basis) mod M,
+ * rho = the group's digit value sum_j d_j p^j.  When ring 0 is such a group, inner_mod is the
+ * whole ring's p^T: ring-0 terms are then multiples of M / inner_mod, the split the kernel
+ * uses. */
 typedef struct es_plan_desc {
   uint32_t abi_version;        /* ES_ABI_VERSION */
   uint32_t k;
@@ -71,7 +79,7 @@ typedef struct es_plan_desc {
   uint32_t lds_budget;         /* max bytes of LDS tables per workgroup (0 = 64 KiB) */
   uint32_t global_masks;       /* masks read through L1 instead of LDS: 0, 2 or 4 (r0 <= 32) */
   uint32_t launch_log2;        /* positions per kernel launch, log2 (0 = 37) */
-  uint32_t reserved;
+  uint32_t inner_mod;          /* modulus of the ring ring 0 is a digit group of (0 = ring_mod[0]) */
 } es_plan_desc;
 
 typedef struct es_engine_info {

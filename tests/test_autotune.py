@@ -26,9 +26,9 @@ def test_search_finds_peak(monkeypatch, method):
     got, trials = AT.autotune(30, probe_log2=12, method=method)
     assert got == best
     if method == "coord":
-        assert len(trials) < 40                  # far fewer probes than the 96-point grid
+        assert len(trials) < 45                  # far fewer probes than the 120-point grid
     else:
-        assert len(trials) == 96 or len(trials) == 48
+        assert len(trials) == 120 or len(trials) == 60
 
 
 def test_coord_keeps_default_within_margin(monkeypatch):

@@ -118,3 +118,64 @@ def test_masks_are_necessary_conditions(k):
     slow = G.slow_order(plan, masks)
     covered = set(slow) | {fi for fi, D, _, _ in masks if D == plan.tp[plan.rest[fi]]}
     assert covered == set(range(len(plan.rest)))
+
+
+def test_fullpower_wheel_k200():
+    """Config-2 wheel: full powers for p <= 23 plus the best single-digit extension; no item
+    left out fits both caps, and no other subset of the fitting items does better (brute force
+    over every subset of the items whose radix fits the R room)."""
+    import itertools
+    import math
+    w = EP.fullpower_wheel(200)
+    full = [(p, len(EP.base_digits(200, p))) for p in EP.primes_upto(23)]
+    assert all(it in w for it in full)
+    P = EP.Plan(200, None, w)
+    assert math.log2(P.M) <= 100 and P.R < (1 << 62)
+    assert len(P.rest) == 37 and sorted(P.rest)[0] == 29
+    R0 = math.prod(EP.partial_count(200, p, T) for p, T in full)
+    M0 = math.prod(p ** T for p, T in full)
+    items = [(p, T) for p in EP.primes_upto(200) if p > 23
+             for T in range(1, len(EP.base_digits(200, p)) + 1)
+             if EP.partial_count(200, p, T) * R0 < (1 << 62)]
+    best = None
+    for r in range(len(items) + 1):
+        for sub in itertools.combinations(items, r):
+            if len({p for p, _ in sub}) < r:
+                continue
+            M = M0 * math.prod(p ** T for p, T in sub)
+            R = R0 * math.prod(EP.partial_count(200, p, T) for p, T in sub)
+            if M <= (1 << 100) and R < (1 << 62):
+                key = (round(math.log2(M / R), 9), -R)
+                if best is None or key > best[0]:
+                    best = (key, sorted(full + list(sub)))
+    assert best[1] == w
+    assert [it for it in w if it[0] > 23] == [(29, 1), (67, 1), (101, 1)]
+
+
+def test_fullpower_wheel_reproduces_config1():
+    """k = 30 with full powers for p <= 13 and M capped at 2^39 is the config-1 wheel."""
+    assert EP.fullpower_wheel(30, p_full=13, m_cap_bits=39) == \
+        [(2, 5), (3, 4), (5, 3), (7, 2), (11, 2), (13, 2)]
+
+
+def test_kernel_digit_rings_keep_idx_order():
+    """Ring 0 of the k=200 full-power wheel (13^3, radix 1056) swept as digit rings: for every
+    slot, the mixed-radix combination of the digit-ring terms (lowest group fastest) is the
+    plan's ring-0 term, and the moduli still multiply to M."""
+    import math
+    import gpu_sieve_mi355x as G
+    P = EP.Plan(200, None, EP.fullpower_wheel(200))
+    mods, radix, terms, inner, f = G.kernel_rings(P, 64)
+    assert P.wheel[0] == (13, 3) and inner == 13 ** 3
+    assert radix[:3] == [8, 11, 12] and f == 132 and math.prod(mods) == P.M
+    assert radix[3:] == P.radix[1:] and terms[3:] == P.ring_terms()[1:]
+    want = P.ring_terms()[0]
+    for slot in range(P.radix[0]):
+        a, b, c = slot % 8, (slot // 8) % 11, slot // 88
+        assert (terms[0][a] + terms[1][b] + terms[2][c]) % P.M == want[slot]
+    M0 = P.M // inner
+    assert all(t % M0 == 0 for t in terms[0])
+    # a wider inner bound groups digits 0 and 1; no split when ring 0 already fits
+    assert G.kernel_rings(P, 128)[1][:2] == [88, 12]
+    P399 = EP.Plan(30)
+    assert G.kernel_rings(P399, 64)[3:] == (0, 1)

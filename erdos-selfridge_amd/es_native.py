@@ -16,6 +16,7 @@ ES_OK, ES_ERR_INVALID, ES_ERR_DEVICE, ES_ERR_OVERFLOW, ES_ERR_LIMIT = 0, -1, -2,
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
 
 
 class PlanDesc(C.Structure):
@@ -87,6 +88,18 @@ def lib() -> C.CDLL:
         "es_fpass_read": (C.c_int, [eng, u32p, C.c_uint64]),
         "es_fpass_read_stream": (C.c_int, [eng, C.c_uint64, C.c_uint64, u64p, u32p]),
         "es_fpass_last_ms": (C.c_int, [eng, C.POINTER(C.c_double)]),
+        "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
+        "es_mlp_destroy": (C.c_int, [eng]),
+        "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),
+        "es_mlp_get_params": (C.c_int, [eng, f32p, C.c_uint32]),
+        "es_mlp_forward": (C.c_int, [eng, f32p, C.c_uint64, f32p]),
+        "es_mlp_gradients": (C.c_int, [eng, f32p, f32p, C.c_uint32, f32p, f32p, f32p]),
+        "es_mlp_adam_step": (C.c_int, [eng, f32p, f32p, C.c_uint32, f32p, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, f32p]),
+        "es_mlp_upload_dataset": (C.c_int, [eng, f32p, f32p, C.c_uint64]),
+        "es_mlp_train": (C.c_int, [eng, C.c_uint32, C.c_uint32, C.c_uint64, f32p, C.c_float,
+                                   C.c_float, C.c_float, C.c_float, f32p,
+                                   C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

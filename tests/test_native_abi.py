@@ -13,12 +13,17 @@ import pytest
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "es_sieve.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                 if f.endswith(".h"))
 
 
-def header_functions(path=HEADER):
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(es_[a-z_]+)\s*\(", text)))
+def header_functions(paths=HEADERS):
+    names = set()
+    for path in paths:
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(es_[a-z_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -27,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     names = header_functions()
     assert len(names) >= 14
     for n in names:
-        assert hasattr(L, n), f"{n} declared in es_sieve.h but not exported"
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
     out = subprocess.run(["nm", "-D", "--defined-only", nat.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     exported = set(re.findall(r"\bT (es_[a-z_]+)\b", out))

@@ -43,12 +43,16 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="batch, k200: autotune the launch shape of each wheel first")
-    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch", "k200"),
+    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch", "k200",
+                                           "surrogate"),
                     default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
                          "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters); "
-                         "batch: k=300..320 batched launches; k200: full-power wheel window")
+                         "batch: k=300..320 batched launches; k200: full-power wheel window; "
+                         "surrogate: learned planner vs plain planner, k=350..400")
     ap.add_argument("--fpass-log2", type=int, default=30)
+    ap.add_argument("--k-step", type=int, default=1, help="surrogate: every k-step-th k")
+    ap.add_argument("--candidates", type=int, default=100_000, help="surrogate: plans per k")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -62,6 +66,8 @@ def main() -> int:
         return bench_batch(a)
     if a.workload == "k200":
         return bench_k200(a)
+    if a.workload == "surrogate":
+        return bench_surrogate(a)
     if a.workload == "copy":
         import torch
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")       # 1 GiB
@@ -351,6 +357,115 @@ def bench_k200(a) -> int:
                      "kernel_avg_launch_ms": dev_ms / max(1, launches),
                      "kernel_positions_per_s": kern_rate},
         "cpu_baseline": cpu}), flush=True)
+    return 0
+
+
+def bench_surrogate(a) -> int:
+    """Learned planner, k = 350..400: train the surrogate on wheels the sieve probes itself,
+    then per k rank the candidate wheels, probe the shortlist and the plain wheel, and compare
+    the chosen wheel with the plain one by the time to exhaust the first nonempty block:
+    (floor(g/M) + 1) * R positions at the wheel's measured rate, where the rate is measured on
+    two 2^36-position windows (block 0, and the window of block floor(g/M) holding g), and
+    both wheels must return g as that window's minimum."""
+    import math
+
+    import numpy as np
+
+    import es_plan
+    import es_surrogate as S
+    import gpu_sieve_mi355x as G
+    bf = {}
+    for line in open(os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt")):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            bf[int(parts[0])] = int(parts[1])
+    ks = list(range(350, 401, a.k_step))
+    SHORTLISTS = (1, 2, 4, 8, 32)
+    t0 = time.perf_counter()
+    X, Y, meta = S.training_set(ks, per_k=24, n_candidates=20000, seed=5)
+    t_data = time.perf_counter() - t0
+    n = len(X)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(n)
+    hold = perm[: max(1, n // 5)]
+    train = perm[max(1, n // 5):]
+    model = S.Surrogate(seed=3)
+    loss, train_ms = model.fit(X[train], Y[train], epochs=400, batch=128, lr=2e-3)
+    pred = model.predict(X[hold])
+
+    def r2(y, p):
+        return float(1 - np.sum((y - p) ** 2) / max(1e-30, np.sum((y - y.mean()) ** 2)))
+    fit = {"samples": n, "holdout": len(hold), "train_ms": train_ms,
+           "r2_log2_density": r2(Y[hold, 0], pred[:, 0]), "r2_log2_ns": r2(Y[hold, 1], pred[:, 1])}
+    model.fit(X, Y, epochs=400, batch=128, lr=2e-3)          # final model on every sample
+    N = 1 << 36
+    rows = []
+    for k in ks:
+        res = S.surrogate_plan(k, model, n_candidates=a.candidates, shortlist=32, probe_log2=34,
+                               seed=k, compare=SHORTLISTS)
+        g = bf[k]
+        out = {}
+        for tag, wheel in (("plain", res["plain"]["wheel"]), ("surrogate", res["best"]["wheel"])):
+            if tag == "surrogate" and res["best"]["plain"]:
+                out[tag] = dict(out["plain"])
+                continue
+            plan = es_plan.Plan(k, None, wheel)
+            eng = G.GpuSieve(plan)
+            gi, gt = plan.n_to_idx(g)
+            lo = max(0, min(plan.R - N, gi - N // 2))
+            eng.scan_window(0, 0, 1 << 26)
+            eng.sync()
+            eng.kernel_time()
+            w0 = time.perf_counter()
+            _, mn0 = eng.scan_window(0, 0, min(N, plan.R))
+            _, mn = eng.scan_window(gt, lo, lo + N)
+            eng.sync()
+            wall = time.perf_counter() - w0
+            assert mn == g, (k, tag, mn)
+            assert mn0 is None or mn0 <= k + 1 or mn0 == g
+            rate = (min(N, plan.R) + N) / wall
+            blocks = g // plan.M + 1
+            out[tag] = {"wheel": [list(x) for x in wheel], "R": plan.R,
+                        "log2_M": math.log2(plan.M), "first_block": int(g // plan.M),
+                        "rate": rate, "time_to_first_block_s": blocks * plan.R / rate,
+                        "expected_s": res["plain" if tag == "plain" else "best"]["cost_s"]}
+        out["k"] = k
+        out["ratio"] = out["surrogate"]["time_to_first_block_s"] / out["plain"]["time_to_first_block_s"]
+        out["expected_ratio"] = out["surrogate"]["expected_s"] / out["plain"]["expected_s"]
+        out["same_wheel"] = bool(res["best"]["plain"])
+        out["sizes"] = res["sizes"]
+        out["analytic_expected_ratio"] = res["sizes"][32]["analytic"]
+        out["rank_s"] = res["rank_seconds"]
+        rows.append(out)
+        print(json.dumps({"k": k, "ratio": round(out["ratio"], 4),
+                          "expected_ratio": round(out["expected_ratio"], 4),
+                          "analytic_expected_ratio": round(out["analytic_expected_ratio"], 4),
+                          "same_wheel": out["same_wheel"]}), file=sys.stderr, flush=True)
+    ratios = np.array([r["ratio"] for r in rows])
+    eratios = np.array([r["expected_ratio"] for r in rows])
+    total_s = sum(r["surrogate"]["time_to_first_block_s"] for r in rows)
+    total_p = sum(r["plain"]["time_to_first_block_s"] for r in rows)
+    print(json.dumps({
+        "metric": "time to first nonempty block, surrogate-planned / plain-planned wheel",
+        "value": total_s / total_p, "unit": "ratio (sum over k)", "n_gpus": 1,
+        "steps": len(ks), "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u128/u32 integer; fp32 network",
+        "data": "b-file g(k); wheels enumerated per k; rates measured on 2^36-position windows",
+        "config": {"workload": f"k={ks[0]}..{ks[-1]} step {a.k_step}, {a.candidates} candidates "
+                               "per k, shortlist 32 + plain, probes of 2^34 positions"},
+        "geomean_ratio": float(np.exp(np.mean(np.log(ratios)))),
+        "geomean_expected_ratio": float(np.exp(np.mean(np.log(eratios)))),
+        "shortlist_sizes": {
+            str(sz): {m: float(np.exp(np.mean(np.log([r["sizes"][sz][m] for r in rows]))))
+                      for m in ("surrogate", "analytic")}
+            for sz in list(SHORTLISTS) + ["top1_no_probe"]},
+        "k_with_other_wheel": int(sum(not r["same_wheel"] for r in rows)),
+        "training": dict(fit, data_seconds=t_data),
+        "per_k": [{"k": r["k"], "ratio": r["ratio"], "expected_ratio": r["expected_ratio"],
+                   "analytic_expected_ratio": r["analytic_expected_ratio"],
+                   "plain_s": r["plain"]["time_to_first_block_s"],
+                   "surrogate_s": r["surrogate"]["time_to_first_block_s"]} for r in rows]}),
+        flush=True)
     return 0
 
 

@@ -18,6 +18,7 @@ steps are kept operation-for-operation alike so plans match the recorded ones bi
 """
 from __future__ import annotations
 
+import functools
 import math
 from itertools import product
 from math import prod
@@ -91,8 +92,10 @@ def is_good(n: int, k: int) -> bool:
     return n > k + 1 and all(kummer_ok(n, k, p) for p in primes_upto(k))
 
 
+@functools.lru_cache(maxsize=512)
 def partial_residues(k: int, p: int, T: int) -> tuple[int, list[int]]:
-    """(p^T, ascending residues r mod p^T whose T low base-p digits dominate k's)."""
+    """(p^T, ascending residues r mod p^T whose T low base-p digits dominate k's).  Memoised
+    (a few k values' tables): callers must not modify the returned list."""
     kd = base_digits(k, p)
     assert 1 <= T <= len(kd)
     # digit i ranges over [kd[i], p); residues = all sums of digit * p^i

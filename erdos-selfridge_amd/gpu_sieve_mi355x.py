@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import functools
 import json
 import math
 import os
@@ -80,6 +81,15 @@ def kernel_rings(plan: Plan, max_inner: int = 64):
         g_terms.append([(r * b0) % M for r in rhos])
     f = radix[0] // g_radix[0]
     return g_mods + mods[1:], g_radix + radix[1:], g_terms + terms[1:], p ** T, f
+
+
+@functools.lru_cache(maxsize=512)
+def _table_words(k: int, p: int, T: int) -> np.ndarray:
+    """pack_bits of the admissible residues mod p^T (memoised; read-only)."""
+    q, res = partial_residues(k, p, T)
+    w = pack_bits(q, res)
+    w.setflags(write=False)
+    return w
 
 
 def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
@@ -163,10 +173,10 @@ class _Desc:
         qs, offs, words = [], [], []
         nw = 0
         for p in self.filters:
-            q, res = plan.info[p]
+            q, _ = plan.info[p]
             qs.append(q)
             offs.append(nw)
-            w = pack_bits(q, res)
+            w = _table_words(k, p, plan.tp[p])
             words.append(w)
             nw += w.size
         self.filt_q = np.asarray(qs or [2], dtype=np.uint32)
@@ -180,8 +190,7 @@ class _Desc:
         nw = 0
         for fi, D, m, _ in self.masks:
             p = plan.rest[fi]
-            _, res = partial_residues(k, p, D)
-            w = pack_bits(m, res)
+            w = _table_words(k, p, D)
             mm.append(m)
             ms.append(fi)
             mo.append(nw)

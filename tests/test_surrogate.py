@@ -1,0 +1,106 @@
+"""Learned planner: vectorised wheel arithmetic against Plan and the engine's kernel geometry
+(CPU); on the GPU, a surrogate trained on wheels the sieve probes itself picks wheels faster
+than the plain planner's where the plain wheel is slow on this kernel, with identical exact
+minima."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import es_plan as EP
+import es_surrogate as S
+import gpu_sieve_mi355x as G
+from conftest import bfile
+
+
+@pytest.mark.parametrize("k", [200, 360, 399, 400])
+def test_plan_space_matches_plan(k):
+    sp = S.PlanSpace(k)
+    E = sp.enumerate(3000, seed=k)
+    assert np.array_equal(E[0], sp.plain())
+    assert len(np.unique(E, axis=0)) == len(E) and sp.valid(E).all()
+    lp = sp.log2_positions(E)
+    assert lp[0] <= lp.min() + 1e-9                 # the plain plan minimises E_pos
+    F = sp.features(E)
+    assert F.shape == (len(E), S.N_FEAT) and np.isfinite(F).all()
+    rng = np.random.default_rng(k)
+    for i in [0] + list(rng.choice(len(E), 6, replace=False)):
+        P = EP.Plan(k, None, sp.wheel(E[i]))
+        assert abs(sp.log2M(E[i:i + 1])[0] - math.log2(P.M)) < 1e-9
+        assert abs(sp.log2R(E[i:i + 1])[0] - math.log2(P.R)) < 1e-9
+        assert abs(2 ** sp.log2_density(E[i:i + 1])[0] / P.density_rest - 1) < 1e-9
+        assert abs(F[i, 60] - P.expected_filter_evals()) < 1e-5
+        assert abs(F[i, S.N_RING_FEAT] * 8 - math.log2(P.radix[0])) < 1e-5
+        assert F[i, 0] * 16 == pytest.approx(math.log2(P.rings[0][0]), abs=1e-5)
+        # kernel geometry: inner radix and strip length as the engine derives them
+        _, kr, _, _, _ = G.kernel_rings(P, 128)
+        assert F[i, 16] * 8 == pytest.approx(math.log2(kr[0]), abs=1e-5)
+        assert F[i, 19] * 16 == pytest.approx(math.log2(_engine_strip(kr)), abs=1e-5)
+
+
+def _engine_strip(radix, target=1024):
+    """The engine's strip rule (es_engine_create): leading radices after ring 0, nearest the
+    target in log scale."""
+    S, m = 1, 0
+    while 1 + m < len(radix):
+        nxt = S * radix[1 + m]
+        if nxt > (1 << 22):
+            break
+        if m > 0 and nxt > target:
+            if nxt * S < target * target:
+                S = nxt
+            break
+        S, m = nxt, m + 1
+    return S
+
+
+def test_plain_vector_roundtrip():
+    sp = S.PlanSpace(380)
+    w = EP.Plan(380).wheel
+    assert sorted(sp.wheel(sp.vector(w))) == sorted(w)
+
+
+@pytest.mark.gpu
+def test_surrogate_pick_beats_plain_with_identical_minimum():
+    """What the trained network gains over the method without it: trained on 80 wheels the
+    sieve probes at k = 380..400, its single top-ranked wheel (no probe) at k = 391 and 400 has
+    a measured expected time to the first nonempty block well below the plain planner's wheel
+    (observed 0.10-0.56 over seeds; bound 0.7), and the chosen and plain wheels return the same
+    exact minimum g(k) on a window around it."""
+    import gpu_sieve_mi355x as G
+    X, Y, meta = S.training_set([380, 385, 390, 395, 400], per_k=16, n_candidates=10000, seed=2)
+    assert len(X) >= 60
+    model = S.Surrogate(seed=2)
+    loss, _ = model.fit(X, Y, epochs=300, batch=32)
+    assert np.isfinite(loss)
+    gs = bfile()
+    for k in (391, 400):
+        res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=4, probe_log2=32, seed=2,
+                               compare=(1, 4))
+        top1 = res["sizes"]["top1_no_probe"]["surrogate"]
+        print(f"k={k} surrogate top-1 / plain = {top1:.3f}; shortlist 4 / plain = "
+              f"{res['sizes'][4]['surrogate']:.3f}", flush=True)
+        assert top1 < 0.7
+        assert res["best"]["cost_s"] <= res["plain"]["cost_s"]
+        for wheel in (res["best"]["wheel"], res["plain"]["wheel"]):
+            plan = EP.Plan(k, None, wheel)
+            eng = G.GpuSieve(plan)
+            gi, gt = plan.n_to_idx(gs[k])
+            lo = max(0, gi - (1 << 30))
+            _, mn = eng.scan_window(gt, lo, min(plan.R, lo + (1 << 31)))
+            assert mn == gs[k]
+
+
+def test_features_follow_digit_ring_split():
+    """k = 200 full-power wheel: ring 0 = 13^3 (radix 1056) is swept as 88 x 12, strip 4320."""
+    sp = S.PlanSpace(200)
+    e = sp.vector(EP.fullpower_wheel(200))
+    f = sp.features(e[None])[0]
+    assert f[16] * 8 == pytest.approx(math.log2(88), abs=1e-5)
+    assert f[17] * 4 == 3 and f[18] == 1
+    P = EP.Plan(200, None, EP.fullpower_wheel(200))
+    _, kr, _, _, _ = G.kernel_rings(P, 128)
+    assert f[19] * 16 == pytest.approx(math.log2(_engine_strip(kr)), abs=1e-5)
+    assert _engine_strip(kr) == 4320

@@ -121,13 +121,16 @@ def main() -> int:
     ops_ref = 12 + 10 * ef                              # reference-algorithm ops per position
     kern_rate = dev_pos / (dev_ms / 1e3)
     F = eng.info.n_masks_used
-    ops_kernel = (11 + 7 * F) / eng.r0                  # this kernel's VALU ops per position
+    # this kernel's VALU lane-ops per position: 55 VALU instructions per lane per outer index
+    # (r0 positions), measured with rocprofv3 SQ_INSTS_VALU for es_sieve_kernel<8, 0, 1> on this
+    # wheel; other instances are not profiled
+    ops_kernel = 55.0 / eng.r0 if (F == 8 and eng.info.words == 1) else None
     achieved = ops_ref * kern_rate
     roofline = {"bound": "valu-int32", "achieved": achieved / 1e12, "peak": INT32_PEAK / 1e12,
                 "unit": "TOP/s", "frac": achieved / INT32_PEAK, "traffic": None,
                 "ops_per_position_reference": round(ops_ref, 3),
-                "kernel_ops_per_position": round(ops_kernel, 3),
-                "kernel_frac": ops_kernel * kern_rate / INT32_PEAK,
+                "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
+                "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
                 "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
                 "kernel_positions_per_s": kern_rate}
 
@@ -345,8 +348,8 @@ def bench_k200(a) -> int:
         "config": {"workload": "k=200 full-power wheel, 2^36 positions at seeded block",
                    "t": t, "idx_lo": lo, "wheel": [list(x) for x in plan.wheel],
                    "R": plan.R, "filters": len(plan.rest), "autotuned": bool(a.autotune),
-                   "tune": {x: plan.tun[x] for x in ("mi_masks", "threads", "mi_strip_target",
-                                                     "mi_inner_radix")},
+                   "tune": {x: plan.tun[x] for x in ("mi_masks", "threads", "mi_strip_target")},
+                   "inner_bound": eng.inner_bound,
                    "kernel_ring0_radix": eng._d.r0, "strip_len": eng.strip_len},
         "survivors": len(ns), "survivor_hash": O.survivor_hash(ns),
         "min": str(mn) if mn is not None else None,
@@ -382,7 +385,8 @@ def bench_surrogate(a) -> int:
     ks = list(range(350, 401, a.k_step))
     SHORTLISTS = (1, 2, 4, 8, 32)
     t0 = time.perf_counter()
-    X, Y, meta = S.training_set(ks, per_k=24, n_candidates=20000, seed=5)
+    X, Y, meta = S.training_set(ks, per_k=24, n_candidates=20000, seed=5,
+                                log=lambda m: print(m, file=sys.stderr, flush=True))
     t_data = time.perf_counter() - t0
     n = len(X)
     rng = np.random.default_rng(7)

@@ -10,8 +10,9 @@
 // n mod q from the value's four 32-bit limbs in FP64 (sum of limb * (2^32k mod q) < 2^52, so
 // it is exact; quotient by one multiply with 1/q and one fused multiply-subtract, corrected
 // by at most one q), and a bit test in the filter's table.  Tables of the most selective
-// filters sit in LDS; the wave stops testing once no lane is alive.  Survivors are compacted
-// with a wave ballot: one atomic per wave, each lane writes at its prefix count.
+// filters sit in LDS.  Candidates surviving the first filters are packed per wave before the
+// rest are tested (es_filter_kernel).  Survivors are compacted with a wave ballot: one atomic
+// per wave, each lane writes at its prefix count.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -39,19 +40,83 @@ __device__ __forceinline__ uint32_t fp_mod(double d0, double d1, double d2, doub
   return (uint32_t)r;
 }
 
+// Filter test of one value (limbs as doubles) against filters [j0, j1); with wave_exit the loop
+// ends once no lane of the wave is alive.
+template <bool WAVE_EXIT>
+__device__ __forceinline__ bool es_filters(double d0, double d1, double d2, double d3, bool alive,
+                                           uint32_t j0, uint32_t j1, const double* __restrict__ recs,
+                                           uint32_t n_lds, const unsigned long long* lbits,
+                                           const unsigned long long* __restrict__ gbits) {
+  for (uint32_t j = j0; j < j1; ++j) {
+    if (WAVE_EXIT && !__any(alive)) break;
+    const double* f = recs + kFiltRecDw / 2 * j;
+    const uint32_t off = reinterpret_cast<const uint32_t*>(f)[10];
+    const uint32_t r = fp_mod(d0, d1, d2, d3, f);
+    const unsigned long long wv = j < n_lds ? lbits[off + (r >> 6)] : gbits[off + (r >> 6)];
+    alive = alive && ((wv >> (r & 63u)) & 1ull);
+  }
+  return alive;
+}
+
+// Survivors of a wave to the output: one atomic per wave, lanes write at their prefix count.
+__device__ __forceinline__ void es_emit(bool alive, uint32_t id, uint32_t* __restrict__ out,
+                                        unsigned long long* __restrict__ out_n, uint64_t out_cap) {
+  const unsigned long long mask = __ballot(alive);
+  if (!mask) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+  unsigned long long slot0 = 0;
+  const uint32_t leader = __ffsll((long long)mask) - 1;
+  if (lane == leader) slot0 = atomicAdd(out_n, (unsigned long long)__popcll(mask));
+  slot0 = __shfl(slot0, leader);
+  if (alive && slot0 + rank < out_cap) out[slot0 + rank] = id;
+}
+
+// Two-stage pass.  Stage 1: every lane tests its UNROLL candidates against the first n_first
+// filters (no early exit: the lanes are dense).  The few candidates left (~6% after two filters
+// at k = 397) are packed into the wave's own 64-entry LDS list (ID and value), and when the
+// next batch would overflow it, stage 2 runs the remaining filters over the list with every
+// lane holding a live candidate, ending once the wave has none left.  A wave stops after a
+// handful of filters instead of testing until the last of 64 sparse lanes dies.  Lists are
+// private to a wave (no workgroup barrier); the flush condition is wave-uniform.
 template <int UNROLL>
 __global__ void __launch_bounds__(256)
 es_filter_kernel(const uint32_t* __restrict__ ids, uint64_t n_ids,
                  const uint4* __restrict__ values, const double* __restrict__ recs,
-                 uint32_t n_filt, uint32_t n_lds, const unsigned long long* __restrict__ gbits,
-                 uint32_t lds_words64, uint32_t* __restrict__ out,
-                 unsigned long long* __restrict__ out_n, uint64_t out_cap) {
+                 uint32_t n_filt, uint32_t n_first, uint32_t n_lds,
+                 const unsigned long long* __restrict__ gbits, uint32_t lds_words64,
+                 uint32_t* __restrict__ out, unsigned long long* __restrict__ out_n,
+                 uint64_t out_cap) {
   extern __shared__ unsigned long long lbits[];
   for (uint32_t z = threadIdx.x; z < lds_words64; z += blockDim.x) lbits[z] = gbits[z];
   __syncthreads();
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  // wave lists after the tables (16-byte aligned): values [nwaves][64], then IDs [nwaves][64]
+  uint4* qv_all = reinterpret_cast<uint4*>(lbits + ((lds_words64 + 1u) & ~1u));
+  uint4* qv = qv_all + wave * 64u;
+  uint32_t* qi = reinterpret_cast<uint32_t*>(qv_all + nwaves * 64u) + wave * 64u;
+  uint32_t cnt = 0;  // entries in this wave's list (wave-uniform)
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool have = lane < cnt;
+    const uint32_t id = have ? qi[lane] : 0u;
+    const uint4 v = have ? qv[lane] : make_uint4(0, 0, 0, 0);
+    const bool alive = es_filters<true>((double)v.x, (double)v.y, (double)v.z, (double)v.w, have,
+                                        n_first, n_filt, recs, n_lds, lbits, gbits);
+    es_emit(alive, id, out, out_n, out_cap);
+    cnt = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * UNROLL;
-  for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x) * UNROLL + threadIdx.x; base < n_ids;
-       base += stride) {
+  // the loop bound is wave-uniform (the wave's first lane), so every lane of a wave runs the
+  // same rounds and the list state stays uniform; lanes past n_ids carry no candidate
+  for (uint64_t base0 = ((uint64_t)blockIdx.x * blockDim.x) * UNROLL + (threadIdx.x & ~63u);
+       base0 < n_ids; base0 += stride) {
+    const uint64_t base = base0 + lane;
     uint32_t id[UNROLL];
     uint4 v[UNROLL];
     bool ok[UNROLL];
@@ -62,39 +127,24 @@ es_filter_kernel(const uint32_t* __restrict__ ids, uint64_t n_ids,
       id[u] = ok[u] ? __builtin_nontemporal_load(ids + i) : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      if (ok[u]) {
-        v[u] = values[id[u]];
-      } else {
-        v[u] = make_uint4(0, 0, 0, 0);
-      }
-    }
+    for (int u = 0; u < UNROLL; ++u) v[u] = ok[u] ? values[id[u]] : make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-      const double d0 = (double)v[u].x, d1 = (double)v[u].y, d2 = (double)v[u].z,
-                   d3 = (double)v[u].w;
-      bool alive = ok[u];
-      for (uint32_t j = 0; j < n_filt; ++j) {
-        if (!__any(alive)) break;
-        const double* f = recs + kFiltRecDw / 2 * j;
-        const uint32_t off = reinterpret_cast<const uint32_t*>(f)[10];
-        const uint32_t r = fp_mod(d0, d1, d2, d3, f);
-        const unsigned long long wv = j < n_lds ? lbits[off + (r >> 6)] : gbits[off + (r >> 6)];
-        alive = alive && ((wv >> (r & 63u)) & 1ull);
-      }
-      // wave compaction: one atomic per wave
+      const bool alive = es_filters<false>((double)v[u].x, (double)v[u].y, (double)v[u].z,
+                                           (double)v[u].w, ok[u], 0, n_first, recs, n_lds, lbits,
+                                           gbits);
       const unsigned long long mask = __ballot(alive);
-      if (mask) {
-        const uint32_t lane = __lane_id();
-        const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-        unsigned long long slot0 = 0;
-        const uint32_t leader = __ffsll((long long)mask) - 1;
-        if (lane == leader) slot0 = atomicAdd(out_n, (unsigned long long)__popcll(mask));
-        slot0 = __shfl(slot0, leader);
-        if (alive && slot0 + rank < out_cap) out[slot0 + rank] = id[u];
+      const uint32_t n = __popcll(mask);
+      if (cnt + n > 64u) flush();
+      if (alive) {
+        const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+        qi[slot] = id[u];
+        qv[slot] = v[u];
       }
+      cnt += n;
     }
   }
+  if (cnt) flush();
 }
 
 // Seeded generator of the value stream: n = (sum_p P_p * ((r_p * inv_p) mod m_p)) mod P + J*P,
@@ -158,6 +208,7 @@ struct es_fpass {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int cus = 256;
+  uint32_t unroll = 8, n_first = 3;  // candidates per lane per round, stage-1 filters
 };
 
 extern "C" {
@@ -327,18 +378,31 @@ int es_fpass_generate(es_fpass* f, uint64_t n, uint64_t seed, uint32_t n_mods,
   return ES_OK;
 }
 
+int es_fpass_set_shape(es_fpass* f, uint32_t unroll, uint32_t n_first) {
+  if (!f) return fail(ES_ERR_INVALID, "null argument");
+  if (unroll != 4 && unroll != 8) return fail(ES_ERR_INVALID, "unroll must be 4 or 8");
+  f->unroll = unroll;
+  f->n_first = n_first;
+  return ES_OK;
+}
+
 int es_fpass_run(es_fpass* f, uint64_t* n_out) {
   if (!f) return fail(ES_ERR_INVALID, "null argument");
   ES_HIP(hipMemsetAsync(f->d_count, 0, 8, f->stream));
-  constexpr int U = 4;
-  const uint64_t per_wg = 256ull * U;
+  const uint32_t U = f->unroll, threads = 256, nwaves = threads / 64;
+  const uint64_t per_wg = (uint64_t)threads * U;
   uint64_t blocks = (f->n_ids + per_wg - 1) / per_wg;
   blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)f->cus * 32));
+  const uint32_t n_first = std::min(f->n_first, f->n_filt);
+  const size_t lds = (size_t)((f->lds_words64 + 1u) & ~1u) * 8 + (size_t)nwaves * 64 * 20;
+  auto kern = U == 8 ? es_filter_kernel<8> : es_filter_kernel<4>;
+  if (lds > 65536)
+    ES_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
   ES_HIP(hipEventRecord(f->ev0, f->stream));
-  hipLaunchKernelGGL(es_filter_kernel<U>, dim3((uint32_t)blocks), dim3(256),
-                     f->lds_words64 * 8, f->stream, f->d_ids, f->n_ids, f->d_values,
-                     (const double*)f->d_recs, f->n_filt, f->n_lds, f->d_bits, f->lds_words64,
-                     f->d_out, f->d_count, f->out_cap);
+  hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(threads), lds, f->stream, f->d_ids,
+                     f->n_ids, f->d_values, (const double*)f->d_recs, f->n_filt, n_first,
+                     f->n_lds, f->d_bits, f->lds_words64, f->d_out, f->d_count, f->out_cap);
   ES_HIP(hipGetLastError());
   ES_HIP(hipEventRecord(f->ev1, f->stream));
   unsigned long long c = 0;

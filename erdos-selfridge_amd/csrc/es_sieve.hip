@@ -187,6 +187,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
     } else {
       // carrying slots are the suffix [j, r0) of the sorted ring-0 slots
       const uint32_t j = reinterpret_cast<const uint16_t*>(carry_tab)[h];
+#pragma unroll
       for (uint32_t w = 0; w < nw; ++w) {
         const int d = (int)j - 32 * (int)w;
         const uint32_t carry = d <= 0 ? 0xffffffffu : (d >= 32 ? 0u : 0xffffffffu << d);
@@ -339,6 +340,8 @@ KernelFn kernel_for(uint32_t W, uint32_t G) {
   }
   if (G != 0) return nullptr;
   if (W == 2) return es_sieve_kernel<F, 0, 2>;
+  if (W == 3) return es_sieve_kernel<F, 0, 3>;
+  if (W == 4) return es_sieve_kernel<F, 0, 4>;
   return es_sieve_kernel<F, 0, 0>;
 }
 
@@ -347,6 +350,8 @@ GridBatchFn grid_batch_for(uint32_t W, uint32_t G) {
   if (G != 0) return nullptr;
   if (W == 1) return es_grid_batch_kernel<F, 0, 1>;
   if (W == 2) return es_grid_batch_kernel<F, 0, 2>;
+  if (W == 3) return es_grid_batch_kernel<F, 0, 3>;
+  if (W == 4) return es_grid_batch_kernel<F, 0, 4>;
   return es_grid_batch_kernel<F, 0, 0>;
 }
 

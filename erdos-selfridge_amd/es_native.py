@@ -88,6 +88,7 @@ def lib() -> C.CDLL:
         "es_fpass_read": (C.c_int, [eng, u32p, C.c_uint64]),
         "es_fpass_read_stream": (C.c_int, [eng, C.c_uint64, C.c_uint64, u64p, u32p]),
         "es_fpass_last_ms": (C.c_int, [eng, C.POINTER(C.c_double)]),
+        "es_fpass_set_shape": (C.c_int, [eng, C.c_uint32, C.c_uint32]),
         "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
         "es_mlp_destroy": (C.c_int, [eng]),
         "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),

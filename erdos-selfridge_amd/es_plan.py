@@ -48,7 +48,8 @@ TUNABLES = {
     "mi_masks": 8,             # word-mask prefilters staged in LDS
     "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
     "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
-    "mi_inner_radix": 128,     # ring 0 mod p^T wider than this is swept as digit rings (0: never)
+    "mi_inner_radix": "auto",  # ring 0 mod p^T wider than this is swept as digit rings (0:
+                               # never; "auto": the engine times the distinct splits)
 }
 
 

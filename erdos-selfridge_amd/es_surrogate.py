@@ -33,7 +33,8 @@ import es_plan as EP
 
 N_FEAT = 64
 N_RING_FEAT = 8      # rings described by (log2 q, log2 radix)
-N_SMALL = 16         # exponent fraction of the 16 smallest primes
+N_SMALL = 8          # exponent fraction of the 8 smallest primes
+SPLIT_BOUNDS = (0, 64, 32, 16)   # digit-split alternatives the engine times (0 = whole ring)
 N_LEAD = 8           # leading filters: pass rate and log2 q
 
 
@@ -57,22 +58,15 @@ class PlanSpace:
                 self.cnt[i, T] = c
                 self.rate[i, T] = 1.0 if T == self.tp[i] else full / (c / p ** T)
         self.lcnt = np.log2(self.cnt)
-        # kernel inner radix of a ring (p, T) placed first: its lowest digit group when the
-        # radix exceeds 128 and T >= 2 (the engine's digit-ring split), else the radix
-        self.inner = self.cnt.copy()
+        # kernel inner radix of a ring (p, T) placed first under split bound b: its lowest
+        # digit group when the radix exceeds b and T >= 2 (the engine's digit-ring split),
+        # else the radix (b = 0: never split)
+        self.inner = self._inner_table(128)
+        self.inner_alt = {b: self._inner_table(b) for b in SPLIT_BOUNDS}
         # word-mask pass rate of p as a filter at wheel depth T: the deepest digit sub-filter
         # mod p^D <= 4096, D > T (1 when there is none)
         self.mrate = np.ones_like(self.cnt)
         for i, p in enumerate(self.ps):
-            digits = EP.base_digits(k, p)
-            for T in range(1, int(self.tp[i]) + 1):
-                if self.cnt[i, T] > 128 and T >= 2:
-                    g = p - digits[0]
-                    for j in range(1, T):
-                        if g * (p - digits[j]) > 128:
-                            break
-                        g *= p - digits[j]
-                    self.inner[i, T] = g
             for T in range(0, int(self.tp[i])):
                 D = T
                 while D + 1 <= int(self.tp[i]) and p ** (D + 1) <= 4096:
@@ -82,6 +76,22 @@ class PlanSpace:
         dens = float(np.prod(self.rate[:, 0]))
         self.log2E = -math.log2(dens)
         self.small = min(N_SMALL, P)
+
+    def _inner_table(self, bound: int) -> np.ndarray:
+        out = self.cnt.copy()
+        if bound <= 0:
+            return out
+        for i, p in enumerate(self.ps):
+            digits = EP.base_digits(self.k, p)
+            for T in range(2, int(self.tp[i]) + 1):
+                if self.cnt[i, T] > bound:
+                    g = p - digits[0]
+                    for j in range(1, T):
+                        if g * (p - digits[j]) > bound:
+                            break
+                        g *= p - digits[j]
+                    out[i, T] = g
+        return out
 
     def cnt_at(self, i: int, D: int) -> float:
         return float(EP.partial_count(self.k, self.ps[i], D))
@@ -191,7 +201,10 @@ class PlanSpace:
                  than 128), words per row, split flag, strip length S (leading radices nearest
                  1024), rings decoded per strip, log2 of the exact-tail fraction left by the 8
                  best word masks, ring count, log2 of a per-position VALU estimate;
-          24-39  exponent fraction T_p / t_p of the 16 smallest primes;
+          24-31  exponent fraction T_p / t_p of the 8 smallest primes;
+          32-39  the engine's split alternatives for ring 0 (whole, digit groups <= 64, 32,
+                 16): log2 inner radix and words per row of each (the engine times them and
+                 keeps the fastest);
           40-47  the 8 least filter pass rates (the filter order's head);
           48-55  the 8 least word-mask pass rates (digit sub-filters with modulus <= 4096);
           56-63  k, log2 M, log2 R, log2 density, expected filter evaluations, filter count,
@@ -238,6 +251,12 @@ class PlanSpace:
         o += 8
         F[:, o:o + self.small] = Ei[:, :self.small] / self.tp[None, :self.small]
         o += N_SMALL
+        T0 = Ei[rows, p0]
+        for b in SPLIT_BOUNDS:
+            rb = self.inner_alt[b][p0, T0]
+            F[:, o] = np.log2(rb) / 8.0
+            F[:, o + 1] = np.ceil(rb / 32.0) / 4.0
+            o += 2
         rate = self._take(self.rate, E)
         isf = Ei < self.tp[None, :]
         rkey = np.where(isf, rate, 2.0)
@@ -291,26 +310,26 @@ class PlanSpace:
 # ----------------------------------------------------------------------------------------- GPU
 
 def probe(k: int, wheel, log2_positions: int = 33, t: int = 1, tun: dict | None = None,
-          repeats: int = 2) -> float | None:
-    """Measured ns per residue position of a wheel (best of `repeats` scans of a window of
-    2^log2_positions in block t, device time); None if the engine rejects the wheel."""
+          repeats: int = 2, max_s: float = 0.05) -> float | None:
+    """Measured ns per residue position of a wheel on windows of block t (device time, best of
+    `repeats`); None if the engine rejects the wheel.  Windows as gpu_sieve_mi355x.timed_rate
+    chooses them (GPU-filling, at most 2^log2_positions positions, shrunk for very slow
+    wheels)."""
     import gpu_sieve_mi355x as G
     try:
         plan = EP.Plan(k, tun, wheel)
         eng = G.GpuSieve(plan)
     except AssertionError:
         return None
-    n = min(1 << log2_positions, plan.R)
-    lo = (plan.R - n) // 2
-    eng.scan_window(t, lo, lo + min(n, 1 << 26))
-    eng.sync()
-    eng.kernel_time()
-    best = math.inf
-    for _ in range(repeats):
-        eng.scan_window(t, lo, lo + n)
+
+    def scan(lo, hi):
+        eng.scan_window(t, lo, hi)
         ms, _, pos = eng.kernel_time()
-        best = min(best, ms * 1e6 / pos)
-    return best
+        return pos / max(ms * 1e-3, 1e-12)
+    sp = eng.info.strip_len * eng.info.r0
+    best = max(G.timed_rate(scan, plan.R, sp, max_s=max_s, hi_log2=log2_positions)
+               for _ in range(repeats))
+    return 1e9 / best
 
 
 class Surrogate:
@@ -342,8 +361,8 @@ class Surrogate:
 def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
                  log=None):
     """Wheels the sieve measures itself: per k, a quarter from the 64 lowest-E_pos candidates,
-    half from the 2000 lowest (the band a shortlist is drawn from) and a quarter uniform over
-    the candidates; labels (log2 density, log2 measured ns per position)."""
+    half from the 2000 lowest (the band a shortlist is drawn from) and a quarter from the 20000
+    lowest; labels (log2 density, log2 measured ns per position)."""
     X, Y, meta = [], [], []
     rng = random.Random(seed)
     for k in ks:
@@ -355,8 +374,9 @@ def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
         band = order[:2000]
         while len(picks) < min(3 * per_k // 4, len(band)):
             picks.add(rng.choice(band))
+        wide = order[:20000]
         while len(picks) < min(per_k, len(E)):
-            picks.add(rng.randrange(len(E)))
+            picks.add(rng.choice(wide))
         picks = sorted(picks)
         feats = sp.features(E[picks])
         dens = sp.log2_density(E[picks])
@@ -368,7 +388,7 @@ def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
             Y.append([dens[j], math.log2(ns)])
             meta.append((k, sp.wheel(E[i]), ns))
         if log:
-            log(f"k={k}: {len(X)} probed wheels so far")
+            log(f"training set k={k}: {len(X)} probed wheels so far")
     return np.asarray(X, dtype=np.float32), np.asarray(Y, dtype=np.float32), meta
 
 

@@ -290,6 +290,28 @@ class _TConst:
 
 # --------------------------------------------------------------------------- engine
 
+def timed_rate(scan, R: int, strip_positions: int, max_s: float = 0.05,
+               strips_log2: int = 19, lo_log2: int = 28, hi_log2: int = 34) -> float:
+    """Positions per second of a layout, from scan(lo, hi) -> rate (device time) on windows
+    centred in [0, R).  The target window holds 2^strips_log2 strips (about 8 workgroups per CU,
+    so long strips are not timed on an under-filled GPU), clamped to [2^lo_log2, 2^hi_log2]
+    positions.  Windows grow from 2^24 positions, each to what the latest rate says fits in
+    max_s, until the target is reached or the next window would not fit; the last rate is
+    returned."""
+    n = max(1, min(R // 2, 1 << hi_log2, max(1 << lo_log2, strip_positions << strips_log2)))
+    w = max(1, min(n, 1 << 24))
+    rate = scan((R - w) // 2, (R - w) // 2 + w)
+    for _ in range(8):
+        if w >= n:
+            break
+        want = int(rate * max_s)
+        if want <= w:
+            break
+        w = min(n, want)
+        rate = scan((R - w) // 2, (R - w) // 2 + w)
+    return rate
+
+
 class GpuSieve:
     """One wheel plan on the current GPU (sieve/gpu_sieve_v9.py:425-568)."""
 
@@ -298,7 +320,9 @@ class GpuSieve:
         P = plan
         self._h = None
         L = nat.lib()
-        self._d = _Desc(P, P.tun)
+        inner = P.tun.get("mi_inner_radix", "auto")
+        self.inner_bound = self._choose_inner(P) if inner == "auto" else int(inner)
+        self._d = _Desc(P, dict(P.tun, mi_inner_radix=self.inner_bound))
         h = C.c_void_p()
         nat.check(L.es_engine_create(C.byref(self._d.desc), C.byref(h)))
         self._h = h
@@ -337,6 +361,52 @@ class GpuSieve:
             except Exception:
                 pass
             self._h = None
+
+    @staticmethod
+    def _choose_inner(P: Plan, strips_log2: int = 19) -> int:
+        """Digit-split bound for a ring 0 wider than 64 slots mod p^T (T >= 2): time every
+        distinct kernel inner radix the split can give (whole ring, groups <= 128, 64, 32, 16)
+        on a window of block 1 and keep the fastest.  Every choice scans the same positions
+        with the same exact test; only the LDS layout (words per row, masks that fit) and so
+        the speed differ.  Each window holds 2^strips_log2 strips of its own layout (about 8
+        workgroups per CU, so long strips are not timed on an under-filled GPU), between 2^30
+        and 2^34 positions."""
+        if not P.rings or P.radix[0] <= 64 or P.wheel[0][1] < 2:
+            return 128
+        opts = {}
+        for bound in (128, 64, 32, 16, 0):
+            opts.setdefault(kernel_rings(P, bound)[1][0], bound)
+        if len(opts) == 1:
+            return 128
+        L = nat.lib()
+        best, best_rate = 128, 0.0
+        for bound in opts.values():
+            try:
+                d = _Desc(P, dict(P.tun, mi_inner_radix=bound))
+            except AssertionError:
+                continue
+            h = C.c_void_p()
+            if L.es_engine_create(C.byref(d.desc), C.byref(h)) != nat.ES_OK:
+                continue
+            try:
+                info = nat.EngineInfo()
+                nat.check(L.es_engine_info_get(h, C.byref(info)))
+
+                def scan(lo, hi):
+                    ms = C.c_double(0)
+                    la, pos = C.c_uint64(0), C.c_uint64(0)
+                    nat.check(L.es_engine_set_block(h, 1))
+                    nat.check(L.es_engine_reset(h))
+                    nat.check(L.es_engine_scan_window(h, lo, hi))
+                    nat.check(L.es_engine_sync(h))
+                    nat.check(L.es_engine_kernel_time(h, C.byref(ms), C.byref(la), C.byref(pos)))
+                    return pos.value / max(ms.value * 1e-3, 1e-12)
+                rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05)
+                if rate > best_rate:
+                    best, best_rate = bound, rate
+            finally:
+                L.es_engine_destroy(h)
+        return best
 
     # -- reference contract ------------------------------------------------------------
     def _launch(self, ostart: int, ocount: int) -> None:

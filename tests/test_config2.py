@@ -94,7 +94,7 @@ def test_c2_digit_rings_dense_parity():
     (_launch in plan outer units) covers the same positions."""
     rng = random.Random(2001)
     t = rng.randrange(1 << 20)
-    split, plan = _engine(mi_filter_limit=6, max_survivors=1 << 22)
+    split, plan = _engine(mi_filter_limit=6, max_survivors=1 << 22, mi_inner_radix=128)
     whole, _ = _engine(mi_filter_limit=6, max_survivors=1 << 22, mi_inner_radix=0)
     assert split._f == 12 and whole._f == 1
     r0 = plan.radix[0]

@@ -140,3 +140,23 @@ def test_gpu_pass_overflow_reported():
     fp.upload(vals, np.arange(1000))
     with pytest.raises(RuntimeError, match="overflow"):
         fp.run()
+
+
+@pytest.mark.gpu
+def test_gpu_pass_shapes_agree_ragged():
+    """Stage-1 depth and unroll change speed only: every shape returns the oracle's survivors,
+    on a ragged stream (length not a multiple of a wave) where stage 1 keeps everything
+    (n_first = 0 sends every candidate through the wave lists), some or few."""
+    prim = FP.filter_order(K, PRIMES[:8])
+    fp = FP.FilterPass(K, prim)
+    spec = FP.generator_spec(K, fp.primes, mod_bits=100)
+    vals = _spec_values(spec, 100_003, 9)
+    ids = np.random.default_rng(10).permutation(len(vals)).astype(np.uint32)
+    print(f"case fpass shapes n={len(vals)} filters={len(prim)}", flush=True)
+    fp.upload(vals, ids)
+    want = np.sort(O.c_filter_values(K, fp.primes, vals, ids))
+    assert len(want) > 1000
+    for unroll in (4, 8):
+        for n_first in (0, 1, 2, 5, 8):
+            fp.set_shape(unroll, n_first)
+            assert np.array_equal(fp.run(), want), (unroll, n_first)

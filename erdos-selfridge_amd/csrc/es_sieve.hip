@@ -1033,6 +1033,8 @@ struct es_batch {
   uint64_t out_cap = 1u << 16;
   std::vector<Job> jobs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* d_tables = nullptr;  // plan views and job table of the last run (grown as needed)
+  size_t tables_bytes = 0;
 };
 
 extern "C" {
@@ -1056,6 +1058,7 @@ int es_batch_destroy(es_batch* b) {
     hipFree(j.dmem);
     hipFree(j.d_out);
   }
+  hipFree(b->d_tables);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -1133,22 +1136,24 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
     }
     groups[g].push_back(i);
   }
-  std::vector<void*> scratch;
-  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  // per group: plan views [plan_off, +n) and job entries [job_off, +n) in one device table,
+  // uploaded before the timed launches
+  struct Launch { uint32_t plan_off, job_off, n, lds_max; uint64_t wgs; };
+  std::vector<Launch> runs;
+  std::vector<HotPlan> plans;
+  std::vector<GridJob> gj;
   for (size_t g = 0; g < groups.size(); ++g) {
     // every job's strips as consecutive workgroups of one launch
     const uint32_t threads = keys[g].threads;
-    std::vector<HotPlan> plans;
-    std::vector<GridJob> gj;
-    uint32_t lds_max = 4;
+    Launch L{(uint32_t)plans.size(), (uint32_t)gj.size(), (uint32_t)groups[g].size(), 4, 0};
     uint64_t wgs = 0;
     for (uint32_t ji : groups[g]) {
       const es_batch::Job& j = b->jobs[ji];
       const es_engine* e = j.e;
       GridJob J{};
-      J.plan = (uint32_t)plans.size();
+      J.plan = (uint32_t)(plans.size() - L.plan_off);
       plans.push_back(j.hp);
-      lds_max = std::max(lds_max, e->lds_bytes);
+      L.lds_max = std::max(L.lds_max, e->lds_bytes);
       J.o_lo = j.idx_lo / e->r0;
       J.o_hi = (j.idx_hi + e->r0 - 1) / e->r0;
       J.u_lo = J.o_lo / e->S;
@@ -1160,30 +1165,41 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
       gj.push_back(J);
     }
     if (wgs > 0x7fffffffull) return fail(ES_ERR_LIMIT, "batch too large for one grid");
-    void *d_plans = nullptr, *d_jobs = nullptr;
-    ES_HIP(hipMalloc(&d_plans, plans.size() * sizeof(HotPlan)));
-    ES_HIP(hipMalloc(&d_jobs, gj.size() * sizeof(GridJob)));
-    scratch.push_back(d_plans);
-    scratch.push_back(d_jobs);
-    ES_HIP(hipMemcpyAsync(d_plans, plans.data(), plans.size() * sizeof(HotPlan),
-                          hipMemcpyHostToDevice, b->stream));
-    ES_HIP(hipMemcpyAsync(d_jobs, gj.data(), gj.size() * sizeof(GridJob), hipMemcpyHostToDevice,
+    L.wgs = wgs;
+    runs.push_back(L);
+  }
+  const size_t plan_bytes = (plans.size() * sizeof(HotPlan) + 255) & ~size_t(255);
+  const size_t need = plan_bytes + gj.size() * sizeof(GridJob);
+  if (need > b->tables_bytes) {
+    if (b->d_tables) ES_HIP(hipFree(b->d_tables));
+    b->d_tables = nullptr;
+    ES_HIP(hipMalloc(&b->d_tables, need));
+    b->tables_bytes = need;
+  }
+  uint8_t* tb = (uint8_t*)b->d_tables;
+  if (!plans.empty()) {
+    ES_HIP(hipMemcpyAsync(tb, plans.data(), plans.size() * sizeof(HotPlan), hipMemcpyHostToDevice,
                           b->stream));
+    ES_HIP(hipMemcpyAsync(tb + plan_bytes, gj.data(), gj.size() * sizeof(GridJob),
+                          hipMemcpyHostToDevice, b->stream));
+  }
+  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  for (size_t g = 0; g < runs.size(); ++g) {
+    const Launch& L = runs[g];
     const GridBatchFn fn = keys[g].fn;
-    if (lds_max > 65536)
+    if (L.lds_max > 65536)
       ES_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds_max));
-    const HotPlan* pp = (const HotPlan*)d_plans;
-    const GridJob* jp = (const GridJob*)d_jobs;
-    uint32_t nj = (uint32_t)gj.size();
+                                 (int)L.lds_max));
+    const HotPlan* pp = (const HotPlan*)tb + L.plan_off;
+    const GridJob* jp = (const GridJob*)(tb + plan_bytes) + L.job_off;
+    uint32_t nj = L.n;
     void* args[] = {&pp, &jp, &nj};
-    ES_HIP(hipLaunchKernel((const void*)fn, dim3((uint32_t)std::max<uint64_t>(wgs, 1)),
-                           dim3(threads), args, lds_max, b->stream));
+    ES_HIP(hipLaunchKernel((const void*)fn, dim3((uint32_t)std::max<uint64_t>(L.wgs, 1)),
+                           dim3(keys[g].threads), args, L.lds_max, b->stream));
     ES_HIP(hipGetLastError());
   }
   ES_HIP(hipEventRecord(b->ev1, b->stream));
   ES_HIP(hipStreamSynchronize(b->stream));
-  for (void* p : scratch) ES_HIP(hipFree(p));
   float t = 0.f;
   ES_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
   if (ms) *ms = t;

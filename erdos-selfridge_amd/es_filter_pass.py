@@ -124,7 +124,7 @@ class FilterPass:
     """One filter set on the current GPU (libes_sieve.so es_fpass_*).  Filters are tested in
     the order given (order="given") or most selective first (order="rate")."""
 
-    def __init__(self, k: int, primes: list[int] | None = None, lds_budget: int = 32768,
+    def __init__(self, k: int, primes: list[int] | None = None, lds_budget: int = 16384,
                  out_cap: int = 1 << 24, order: str = "rate"):
         self.k = k
         if primes is None:
@@ -177,7 +177,7 @@ class FilterPass:
         nat.check(nat.lib().es_fpass_read(self._h, nat.ptr32(ids), n.value))
         return np.sort(ids[:n.value])
 
-    def set_shape(self, unroll: int = 8, n_first: int = 3) -> None:
+    def set_shape(self, unroll: int = 2, n_first: int = 3) -> None:
         """Launch shape (speed only): candidates per lane per round, leading filters tested on
         every candidate before a wave packs its survivors for the rest."""
         nat.check(nat.lib().es_fpass_set_shape(self._h, unroll, n_first))

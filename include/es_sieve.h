@@ -181,9 +181,9 @@ int es_fpass_generate(es_fpass* f, uint64_t n, uint64_t seed, uint32_t n_mods,
                       uint64_t j_span);
 /* Run the pass (synchronous); *n_out = survivors (may exceed out_cap: ES_ERR_OVERFLOW). */
 int es_fpass_run(es_fpass* f, uint64_t* n_out);
-/* Launch shape of es_fpass_run: candidates per lane per round (4 or 8) and the number of
+/* Launch shape of es_fpass_run: candidates per lane per round (2, 4 or 8) and the number of
  * leading filters every candidate is tested against before the survivors of a wave are packed
- * for the remaining filters (default 8 and 3).  Speed only: the survivor set is the same. */
+ * for the remaining filters (default 2 and 3).  Speed only: the survivor set is the same. */
 int es_fpass_set_shape(es_fpass* f, uint32_t unroll, uint32_t n_first);
 int es_fpass_read(es_fpass* f, uint32_t* ids, uint64_t count);
 int es_fpass_read_stream(es_fpass* f, uint64_t offset, uint64_t count, uint64_t* values_lohi,

@@ -156,7 +156,7 @@ def test_gpu_pass_shapes_agree_ragged():
     fp.upload(vals, ids)
     want = np.sort(O.c_filter_values(K, fp.primes, vals, ids))
     assert len(want) > 1000
-    for unroll in (4, 8):
+    for unroll in (2, 4, 8):
         for n_first in (0, 1, 2, 5, 8):
             fp.set_shape(unroll, n_first)
             assert np.array_equal(fp.run(), want), (unroll, n_first)

@@ -278,6 +278,32 @@ def bench_batch(a) -> int:
     for jid, want in zip(ids, single):
         assert b.collect(jid) == want
     pos = N * len(jobs)
+    # block sweep: 21 jobs (one 2^28-position window per plan) moved through 16 blocks, one
+    # batched run per block (eager) versus one graph replay per block
+    sw = es_batch.Batch()
+    sw_jobs = []
+    for k, eng in engines.items():
+        lo = rng.randrange(eng.plan.R - (1 << 28))
+        sw_jobs.append(sw.add(eng, 0, lo, lo + (1 << 28)))
+    blocks = [rng.randrange(1 << 20) for _ in range(16)]
+    sw.run()
+    sweep = {}
+    for mode in ("eager", "graph"):
+        if mode == "graph":
+            sw.capture()
+            sw.replay()
+        dev = 0.0
+        w0 = _t.perf_counter()
+        res = []
+        for t in blocks:
+            sw.set_blocks([t] * len(sw_jobs))
+            dev += sw.run()[0] if mode == "eager" else sw.replay()
+            res.append([len(sw.collect(j)) for j in sw_jobs])
+        wall = _t.perf_counter() - w0
+        sweep[mode] = {"device_ms": dev, "wall_s": wall,
+                       "positions_per_s_wall": len(blocks) * len(sw_jobs) * (1 << 28) / wall,
+                       "survivors": sum(map(sum, res))}
+    assert sweep["eager"]["survivors"] == sweep["graph"]["survivors"]
     print(json.dumps({
         "metric": "batched residue positions sieved/s (k=300..320, 4 blocks each)",
         "value": pos / (bms / 1e3), "unit": "positions/s", "n_gpus": 1, "steps": a.steps,
@@ -287,6 +313,8 @@ def bench_batch(a) -> int:
                    "launches": launches, "autotuned": bool(a.autotune),
                    "tuned": {str(k): v for k, v in tuned.items()}},
         "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
+        "block_sweep": {"jobs": len(sw_jobs), "blocks": len(blocks), "window": 1 << 28,
+                        **sweep},
         "survivors_total": sum(len(x) for x in single)}), flush=True)
     return 0
 

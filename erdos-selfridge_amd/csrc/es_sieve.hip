@@ -1025,17 +1025,53 @@ struct es_batch {
     es_engine* e = nullptr;
     uint64_t t = 0, idx_lo = 0, idx_hi = 0, cap = 0;
     void* dmem = nullptr;                   // mask records, slow records, cold plan, counter
+    size_t o_mi = 0, o_si = 0;              // offsets of the mask and slow records in dmem
     unsigned long long* d_out = nullptr;
     unsigned long long* d_count = nullptr;
     HotPlan hp{};
   };
   hipStream_t stream = nullptr;
+  // groups of different shapes are independent: they run concurrently on side streams
+  // (default priority), forked from and joined back to `stream` with events
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kSide] = {nullptr, nullptr, nullptr};
   uint64_t out_cap = 1u << 16;
   std::vector<Job> jobs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* d_tables = nullptr;  // plan views and job table of the last run (grown as needed)
   size_t tables_bytes = 0;
+  size_t tables_plan_bytes = 0, tables_job_bytes = 0;
+  hipGraph_t graph = nullptr;          // captured launches (es_batch_capture)
+  hipGraphExec_t graph_exec = nullptr;
+  void drop_graph() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    if (graph) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
 };
+
+// Zero the survivor counters of a captured batch (a kernel node: graphs here hold no memset
+// nodes).
+__global__ void es_zero_counters_kernel(unsigned long long* const* __restrict__ ptrs, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) *ptrs[i] = 0ull;
+}
+
+// Block constants of a job: the mask and slow-filter records of engine e with t*M mod q.
+static void job_records(const es_engine* e, uint64_t t, std::vector<uint32_t>& mi,
+                        std::vector<uint32_t>& si) {
+  std::vector<uint32_t> tc(std::max<uint32_t>(e->n_filters, 1), 0);
+  for (uint32_t f = 0; f < e->n_filters; ++f) {
+    const uint32_t q = e->filt_q[f];
+    tc[f] = (uint32_t)(((uint64_t)(t % q) * mod_u128(e->M, q)) % q);
+  }
+  mi = e->mask_info;
+  si = e->slow_info;
+  for (uint32_t f = 0; f < e->F; ++f) mi[kMaskInfo * f + 7] = tc[e->mask_src[f]] % e->mask_mod[f];
+  for (uint32_t s = 0; s < e->n_slow; ++s) si[kSlowInfo * s + 5] = tc[e->slow_order[s]];
+}
 
 extern "C" {
 
@@ -1045,6 +1081,11 @@ int es_batch_create(uint64_t out_cap_per_job, es_batch** out) {
   std::unique_ptr<es_batch> b(new es_batch());
   b->out_cap = out_cap_per_job ? out_cap_per_job : (1u << 16);
   ES_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  for (int i = 0; i < es_batch::kSide; ++i) {
+    ES_HIP(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+    ES_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+  }
+  ES_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   ES_HIP(hipEventCreate(&b->ev0));
   ES_HIP(hipEventCreate(&b->ev1));
   *out = b.release();
@@ -1059,8 +1100,17 @@ int es_batch_destroy(es_batch* b) {
     hipFree(j.d_out);
   }
   hipFree(b->d_tables);
+  b->drop_graph();
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  for (int i = 0; i < es_batch::kSide; ++i) {
+    if (b->side[i]) {
+      hipStreamSynchronize(b->side[i]);
+      hipStreamDestroy(b->side[i]);
+    }
+    if (b->join[i]) hipEventDestroy(b->join[i]);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return ES_OK;
@@ -1077,18 +1127,13 @@ int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_
   j.idx_lo = idx_lo;
   j.idx_hi = idx_hi;
   j.cap = b->out_cap;
-  // block constants of this job
-  std::vector<uint32_t> tc(std::max<uint32_t>(e->n_filters, 1), 0);
-  for (uint32_t f = 0; f < e->n_filters; ++f) {
-    const uint32_t q = e->filt_q[f];
-    tc[f] = (uint32_t)(((uint64_t)(t % q) * mod_u128(e->M, q)) % q);
-  }
-  std::vector<uint32_t> mi(e->mask_info), si(e->slow_info);
-  for (uint32_t f = 0; f < e->F; ++f) mi[kMaskInfo * f + 7] = tc[e->mask_src[f]] % e->mask_mod[f];
-  for (uint32_t s = 0; s < e->n_slow; ++s) si[kSlowInfo * s + 5] = tc[e->slow_order[s]];
+  std::vector<uint32_t> mi, si;
+  job_records(e, t, mi, si);
   Blob bl;
   const size_t o_mi = bl.put(mi.data(), mi.size() * 4);
   const size_t o_si = bl.put(si.data(), si.size() * 4);
+  j.o_mi = o_mi;
+  j.o_si = o_si;
   const size_t o_cold = bl.put(nullptr, sizeof(ColdPlan));
   const size_t o_cnt = bl.put(nullptr, 8);
   ES_HIP(hipMalloc(&j.dmem, bl.bytes.size()));
@@ -1107,12 +1152,52 @@ int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_
   j.hp.cold = (const ColdPlan*)(base + o_cold);
   if (job_id) *job_id = (uint32_t)b->jobs.size();
   b->jobs.push_back(j);
+  b->drop_graph();  // a captured graph no longer covers every job
   return ES_OK;
 }
 
-int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
-  if (!b) return fail(ES_ERR_INVALID, "null argument");
-  for (auto& j : b->jobs) ES_HIP(hipMemsetAsync(j.d_count, 0, 8, b->stream));
+int es_batch_set_blocks(es_batch* b, const uint64_t* t, uint32_t n_jobs) {
+  if (!b || (n_jobs && !t) || n_jobs != b->jobs.size())
+    return fail(ES_ERR_INVALID, "one block per job expected");
+  // all records staged in one host buffer, copied in order, one synchronisation
+  std::vector<std::vector<uint32_t>> stage(2 * (size_t)n_jobs);
+  for (uint32_t i = 0; i < n_jobs; ++i) {
+    es_batch::Job& j = b->jobs[i];
+    job_records(j.e, t[i], stage[2 * i], stage[2 * i + 1]);
+    uint8_t* base = (uint8_t*)j.dmem;
+    ES_HIP(hipMemcpyAsync(base + j.o_mi, stage[2 * i].data(), stage[2 * i].size() * 4,
+                          hipMemcpyHostToDevice, b->stream));
+    ES_HIP(hipMemcpyAsync(base + j.o_si, stage[2 * i + 1].data(), stage[2 * i + 1].size() * 4,
+                          hipMemcpyHostToDevice, b->stream));
+  }
+  ES_HIP(hipStreamSynchronize(b->stream));
+  for (uint32_t i = 0; i < n_jobs; ++i) b->jobs[i].t = t[i];
+  return ES_OK;
+}
+
+int es_batch_set_block(es_batch* b, uint32_t job, uint64_t t) {
+  if (!b || job >= b->jobs.size()) return fail(ES_ERR_INVALID, "bad job");
+  es_batch::Job& j = b->jobs[job];
+  std::vector<uint32_t> mi, si;
+  job_records(j.e, t, mi, si);
+  uint8_t* base = (uint8_t*)j.dmem;
+  // ordered after earlier work on the batch's stream (replays read these records)
+  ES_HIP(hipMemcpyAsync(base + j.o_mi, mi.data(), mi.size() * 4, hipMemcpyHostToDevice, b->stream));
+  ES_HIP(hipMemcpyAsync(base + j.o_si, si.data(), si.size() * 4, hipMemcpyHostToDevice, b->stream));
+  ES_HIP(hipStreamSynchronize(b->stream));
+  j.t = t;
+  return ES_OK;
+}
+
+namespace {
+
+// Launch groups of a batch with their device tables uploaded (prepare_batch), and the launch
+// of every group on the batch's stream (launch_batch: no allocation or host synchronisation,
+// so it can be captured into a graph).
+struct BatchLaunch { GridBatchFn fn; uint32_t threads, plan_off, job_off, n, lds_max; uint64_t wgs; };
+
+int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
+  out.clear();
   // Group jobs by kernel instance, workgroup size and occupancy class: a launch allocates its
   // largest plan's LDS for every workgroup, so only plans that fit as many workgroups per CU
   // (2048 lanes and 160 KiB of LDS per CU) share one; a small plan is never held to a large
@@ -1137,7 +1222,7 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
     groups[g].push_back(i);
   }
   // per group: plan views [plan_off, +n) and job entries [job_off, +n) in one device table,
-  // uploaded before the timed launches
+  // then the jobs' counter addresses
   struct Launch { uint32_t plan_off, job_off, n, lds_max; uint64_t wgs; };
   std::vector<Launch> runs;
   std::vector<HotPlan> plans;
@@ -1169,8 +1254,12 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
     runs.push_back(L);
   }
   const size_t plan_bytes = (plans.size() * sizeof(HotPlan) + 255) & ~size_t(255);
-  const size_t need = plan_bytes + gj.size() * sizeof(GridJob);
+  const size_t job_bytes = (gj.size() * sizeof(GridJob) + 255) & ~size_t(255);
+  std::vector<unsigned long long*> ctrs;
+  for (auto& j : b->jobs) ctrs.push_back(j.d_count);
+  const size_t need = plan_bytes + job_bytes + std::max<size_t>(ctrs.size(), 1) * 8;
   if (need > b->tables_bytes) {
+    b->drop_graph();
     if (b->d_tables) ES_HIP(hipFree(b->d_tables));
     b->d_tables = nullptr;
     ES_HIP(hipMalloc(&b->d_tables, need));
@@ -1182,28 +1271,107 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
                           b->stream));
     ES_HIP(hipMemcpyAsync(tb + plan_bytes, gj.data(), gj.size() * sizeof(GridJob),
                           hipMemcpyHostToDevice, b->stream));
+    ES_HIP(hipMemcpyAsync(tb + plan_bytes + job_bytes, ctrs.data(), ctrs.size() * 8,
+                          hipMemcpyHostToDevice, b->stream));
   }
-  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  ES_HIP(hipStreamSynchronize(b->stream));
   for (size_t g = 0; g < runs.size(); ++g) {
     const Launch& L = runs[g];
-    const GridBatchFn fn = keys[g].fn;
     if (L.lds_max > 65536)
-      ES_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)L.lds_max));
-    const HotPlan* pp = (const HotPlan*)tb + L.plan_off;
-    const GridJob* jp = (const GridJob*)(tb + plan_bytes) + L.job_off;
-    uint32_t nj = L.n;
-    void* args[] = {&pp, &jp, &nj};
-    ES_HIP(hipLaunchKernel((const void*)fn, dim3((uint32_t)std::max<uint64_t>(L.wgs, 1)),
-                           dim3(keys[g].threads), args, L.lds_max, b->stream));
+      ES_HIP(hipFuncSetAttribute((const void*)keys[g].fn,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_max));
+    out.push_back({keys[g].fn, keys[g].threads, L.plan_off, L.job_off, L.n, L.lds_max, L.wgs});
+  }
+  b->tables_plan_bytes = plan_bytes;
+  b->tables_job_bytes = job_bytes;
+  return ES_OK;
+}
+
+int launch_batch(es_batch* b, const std::vector<BatchLaunch>& runs, bool zero_kernel) {
+  uint8_t* tb = (uint8_t*)b->d_tables;
+  const uint32_t nj = (uint32_t)b->jobs.size();
+  if (zero_kernel && nj) {
+    unsigned long long* const* pp =
+        (unsigned long long* const*)(tb + b->tables_plan_bytes + b->tables_job_bytes);
+    hipLaunchKernelGGL(es_zero_counters_kernel, dim3((nj + 255) / 256), dim3(256), 0, b->stream,
+                       pp, nj);
     ES_HIP(hipGetLastError());
   }
+  // group g runs on stream g mod (kSide + 1): 0 = the batch's stream, 1.. = side streams
+  const int ns = (int)std::min<size_t>(runs.size(), es_batch::kSide + 1);
+  if (ns > 1) {
+    ES_HIP(hipEventRecord(b->fork, b->stream));
+    for (int i = 0; i + 1 < ns; ++i) ES_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
+  }
+  for (size_t g = 0; g < runs.size(); ++g) {
+    const BatchLaunch& L = runs[g];
+    const int si = (int)(g % (size_t)ns);
+    hipStream_t st = si == 0 ? b->stream : b->side[si - 1];
+    const HotPlan* pp = (const HotPlan*)tb + L.plan_off;
+    const GridJob* jp = (const GridJob*)(tb + b->tables_plan_bytes) + L.job_off;
+    uint32_t n = L.n;
+    void* args[] = {&pp, &jp, &n};
+    ES_HIP(hipLaunchKernel((const void*)L.fn, dim3((uint32_t)std::max<uint64_t>(L.wgs, 1)),
+                           dim3(L.threads), args, L.lds_max, st));
+    ES_HIP(hipGetLastError());
+  }
+  for (int i = 0; i + 1 < ns; ++i) {
+    ES_HIP(hipEventRecord(b->join[i], b->side[i]));
+    ES_HIP(hipStreamWaitEvent(b->stream, b->join[i], 0));
+  }
+  return ES_OK;
+}
+
+}  // namespace
+
+int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
+  if (!b) return fail(ES_ERR_INVALID, "null argument");
+  std::vector<BatchLaunch> runs;
+  int rc = prepare_batch(b, runs);
+  if (rc) return rc;
+  for (auto& j : b->jobs) ES_HIP(hipMemsetAsync(j.d_count, 0, 8, b->stream));
+  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  if ((rc = launch_batch(b, runs, false))) return rc;
   ES_HIP(hipEventRecord(b->ev1, b->stream));
   ES_HIP(hipStreamSynchronize(b->stream));
   float t = 0.f;
   ES_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
   if (ms) *ms = t;
-  if (launches) *launches = (uint32_t)groups.size();
+  if (launches) *launches = (uint32_t)runs.size();
+  return ES_OK;
+}
+
+int es_batch_capture(es_batch* b, uint32_t* launches) {
+  if (!b) return fail(ES_ERR_INVALID, "null argument");
+  std::vector<BatchLaunch> runs;
+  int rc = prepare_batch(b, runs);
+  if (rc) return rc;
+  b->drop_graph();
+  ES_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  rc = launch_batch(b, runs, true);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  ES_HIP(end);
+  b->graph = g;
+  ES_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  if (launches) *launches = (uint32_t)runs.size();
+  return ES_OK;
+}
+
+int es_batch_replay(es_batch* b, double* ms) {
+  if (!b) return fail(ES_ERR_INVALID, "null argument");
+  if (!b->graph_exec) return fail(ES_ERR_INVALID, "no captured graph (es_batch_capture after the last es_batch_add)");
+  ES_HIP(hipEventRecord(b->ev0, b->stream));
+  ES_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  ES_HIP(hipEventRecord(b->ev1, b->stream));
+  ES_HIP(hipStreamSynchronize(b->stream));
+  float t = 0.f;
+  ES_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+  if (ms) *ms = t;
   return ES_OK;
 }
 

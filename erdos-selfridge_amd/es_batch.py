@@ -48,6 +48,42 @@ class Batch:
         nat.check(nat.lib().es_batch_run(self._h, C.byref(ms), C.byref(la)))
         return ms.value, la.value
 
+    def capture(self) -> int:
+        """Record the counter reset and every launch of the current jobs into a hipGraph;
+        returns the number of kernel launches captured."""
+        la = C.c_uint32(0)
+        nat.check(nat.lib().es_batch_capture(self._h, C.byref(la)))
+        return la.value
+
+    def set_block(self, job: int, t: int) -> None:
+        """Move a job to block t (its window of positions is kept)."""
+        nat.check(nat.lib().es_batch_set_block(self._h, job, int(t)))
+        s, _, lo, hi = self.jobs[job]
+        self.jobs[job] = (s, t, lo, hi)
+
+    def set_blocks(self, ts) -> None:
+        """Move every job i to block ts[i] in one upload."""
+        arr = np.ascontiguousarray(np.asarray(ts, dtype=np.uint64))
+        assert arr.size == len(self.jobs)
+        nat.check(nat.lib().es_batch_set_blocks(self._h, nat.ptr64(arr), int(arr.size)))
+        self.jobs = [(s, int(t), lo, hi) for (s, _, lo, hi), t in zip(self.jobs, arr)]
+
+    def replay(self) -> float:
+        """Run the captured graph; device ms."""
+        ms = C.c_double(0)
+        nat.check(nat.lib().es_batch_replay(self._h, C.byref(ms)))
+        return ms.value
+
+    def sweep(self, blocks) -> list[list[list[int]]]:
+        """Survivors of every job for each block t in `blocks` (every job moved to t, then one
+        graph replay per block): result[i][job] for blocks[i]."""
+        out = []
+        for t in blocks:
+            self.set_blocks([t] * len(self.jobs))
+            self.replay()
+            out.append([self.collect(j) for j in range(len(self.jobs))])
+        return out
+
     def collect(self, job: int) -> list[int]:
         n = C.c_uint64(0)
         nat.check(nat.lib().es_batch_collect(self._h, job, None, 0, C.byref(n)))

@@ -78,6 +78,10 @@ def lib() -> C.CDLL:
         "es_batch_add": (C.c_int, [eng, eng, C.c_uint64, C.c_uint64, C.c_uint64, u32p]),
         "es_batch_run": (C.c_int, [eng, C.POINTER(C.c_double), u32p]),
         "es_batch_collect": (C.c_int, [eng, C.c_uint32, u64p, C.c_uint64, u64p]),
+        "es_batch_capture": (C.c_int, [eng, u32p]),
+        "es_batch_set_block": (C.c_int, [eng, C.c_uint32, C.c_uint64]),
+        "es_batch_set_blocks": (C.c_int, [eng, u64p, C.c_uint32]),
+        "es_batch_replay": (C.c_int, [eng, C.POINTER(C.c_double)]),
         "es_fpass_create": (C.c_int, [C.c_uint32, u32p, u32p, u64p, C.c_uint32, C.c_uint32,
                                       C.c_uint64, C.POINTER(eng)]),
         "es_fpass_destroy": (C.c_int, [eng]),

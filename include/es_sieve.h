@@ -156,6 +156,16 @@ int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_
 /* Synchronous; *ms = device time of all launches, *launches = launches issued. */
 int es_batch_run(es_batch* b, double* ms, uint32_t* launches);
 int es_batch_collect(es_batch* b, uint32_t job, uint64_t* ids, uint64_t cap, uint64_t* n);
+/* Block sweeps through a captured graph: es_batch_capture records the counter reset (a kernel
+ * node) and every launch of the current jobs into a hipGraph; es_batch_set_block moves a job to
+ * another block t (rewrites its block constants in device memory, outside the graph);
+ * es_batch_replay runs the graph (synchronous, *ms = device time).  Adding a job drops the
+ * graph.  No reference counterpart: the reference relaunches per chunk from Python. */
+int es_batch_capture(es_batch* b, uint32_t* launches);
+int es_batch_set_block(es_batch* b, uint32_t job, uint64_t t);
+/* Every job at once: t[i] for job i (n_jobs = number of jobs), one synchronisation. */
+int es_batch_set_blocks(es_batch* b, const uint64_t* t, uint32_t n_jobs);
+int es_batch_replay(es_batch* b, double* ms);
 
 /* ---- Candidate filter pass over a 128-bit value stream (HBM-bound) ------------------------
  * Reference interface replaced: the late scalar filters the word-mask kernel runs on the

@@ -333,29 +333,36 @@ def probe(k: int, wheel, log2_positions: int = 33, t: int = 1, tun: dict | None 
 
 
 class Surrogate:
-    """Feature/label scalers plus the GPU network."""
+    """Feature/label scalers plus an ensemble of GPU networks (predictions averaged): the
+    networks differ in initialisation and minibatch order, and averaging them steadies the
+    ranking the planner takes from them."""
 
-    def __init__(self, seed: int = 0):
+    def __init__(self, seed: int = 0, ensemble: int = 3):
         import es_mlp
-        self.net = es_mlp.Mlp(batch_cap=8192, seed=seed)
+        self.nets = [es_mlp.Mlp(batch_cap=8192, seed=seed * 97 + i) for i in range(ensemble)]
+        self.seed = seed
         self.fmu = np.zeros(N_FEAT, dtype=np.float32)
         self.fsd = np.ones(N_FEAT, dtype=np.float32)
         self.ymu = np.zeros(2, dtype=np.float32)
         self.ysd = np.ones(2, dtype=np.float32)
 
     def fit(self, X, Y, epochs: int = 300, batch: int = 128, lr: float = 2e-3, seed: int = 1):
+        """Train every network; returns (mean final minibatch loss, total device ms)."""
         X = np.asarray(X, dtype=np.float32)
         Y = np.asarray(Y, dtype=np.float32)
         self.fmu, self.fsd = X.mean(0), X.std(0) + 1e-6
         self.ymu, self.ysd = Y.mean(0), Y.std(0) + 1e-6
         Xn = (X - self.fmu) / self.fsd
         Yn = (Y - self.ymu) / self.ysd
-        return self.net.fit(Xn, Yn, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        out = [net.fit(Xn, Yn, epochs=epochs, batch=batch, lr=lr, seed=seed + 7919 * i)
+               for i, net in enumerate(self.nets)]
+        return float(np.mean([o[0] for o in out])), float(sum(o[1] for o in out))
 
     def predict(self, X) -> np.ndarray:
         """[n, 2]: log2 survivor density per position, log2 ns per position."""
         Xn = (np.asarray(X, dtype=np.float32) - self.fmu) / self.fsd
-        return self.net.forward(Xn) * self.ysd + self.ymu
+        y = np.mean([net.forward(Xn) for net in self.nets], axis=0)
+        return y * self.ysd + self.ymu
 
 
 def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,

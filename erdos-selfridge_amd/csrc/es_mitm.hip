@@ -1,0 +1,429 @@
+// es_mitm.hip -- bounded n-window enumeration by meet in the middle (sieve/gpu_mitm.py:32-125).
+//
+// The wheel's two halves: half 1 (decoded per index in the kernel) and half 2 (a table of all
+// its CRT partial sums mod M, built and sorted on the device: one decode kernel, then two
+// stable 64-bit radix sorts, low word then high word, giving 128-bit order).  For a window
+// [a, b) of s = n - t*M, the half-2 sums meeting s1 lie on the circular run
+// [(a - s1) mod M, +(b - a)) of the sorted table: two lower-bound searches give its ends.
+// Every n formed is tested against the filter tables (n mod q from 128-bit limbs and
+// (2^32k mod q), plus t*M mod q), with early exit; survivors are appended.
+//
+// Lanes: one lane per s1 when runs are short (most lanes only search), or one wave per s1 when
+// runs are long (the wave walks the run 64 entries at a time, coalesced table reads).
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "es_common.h"
+#include "../../include/es_mitm.h"
+
+using es::fail;
+using es::mod_u128;
+
+namespace {
+
+constexpr int kFiltRec = 8;  // q, bitmap word offset, 2^32 / 2^64 / 2^96 mod q, t*M mod q, pad
+
+struct MitmArgs {
+  uint64_t M_lo, M_hi, a_lo, a_hi, len_lo, len_hi;  // window [a, a + len) of s, len <= M
+  uint32_t w1, nf;
+  const uint32_t* radix1;
+  const uint32_t* off1;
+  const uint64_t* t1lo;
+  const uint64_t* t1hi;
+  uint64_t n2;
+  const uint64_t* s2hi;
+  const uint64_t* s2lo;
+  const uint32_t* frec;
+  const uint32_t* fbits;
+  unsigned long long* out;       // survivors: lo, hi pairs
+  unsigned long long* out_n;
+  unsigned long long* tested;    // candidates formed
+  uint64_t out_cap;
+};
+
+__device__ __forceinline__ u128 mitm_s1(const MitmArgs& A, uint64_t idx, u128 M) {
+  u128 s = 0;
+  uint64_t rem = idx;
+  for (uint32_t j = 0; j < A.w1; ++j) {
+    const uint32_t r = A.radix1[j];
+    uint32_t d;
+    if (j + 1 == A.w1) {
+      d = (uint32_t)rem;
+    } else {
+      const uint64_t q = rem / r;
+      d = (uint32_t)(rem - q * r);
+      rem = q;
+    }
+    const uint32_t o = A.off1[j] + d;
+    s += es::mk128(A.t1lo[o], A.t1hi[o]);
+    if (s >= M) s -= M;
+  }
+  return s;
+}
+
+// first index u in [0, n2) with table[u] >= x (128-bit order)
+__device__ __forceinline__ uint64_t mitm_lower_bound(const MitmArgs& A, u128 x) {
+  uint64_t lo = 0, hi = A.n2;
+  const uint64_t xh = (uint64_t)(x >> 64), xl = (uint64_t)x;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    const uint64_t h = A.s2hi[mid], l = A.s2lo[mid];
+    if (h < xh || (h == xh && l < xl)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool mitm_test(const MitmArgs& A, u128 s) {
+  const uint64_t lo = (uint64_t)s, hi = (uint64_t)(s >> 64);
+  for (uint32_t j = 0; j < A.nf; ++j) {
+    const uint32_t* f = A.frec + kFiltRec * j;
+    const uint32_t q = f[0];
+    uint32_t r = es_mod128(lo, hi, q, f[2], f[3], f[4]) + f[5];
+    if (r >= q) r -= q;
+    if (!((A.fbits[f[1] + (r >> 5)] >> (r & 31u)) & 1u)) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void mitm_emit(const MitmArgs& A, u128 s) {
+  const unsigned long long slot = atomicAdd(A.out_n, 1ull);
+  if (slot < A.out_cap) {
+    A.out[2 * slot] = (uint64_t)s;
+    A.out[2 * slot + 1] = (uint64_t)(s >> 64);
+  }
+}
+
+// The circular run of table indices meeting s1: [r0s, r0e) and [r1s, r1e).
+__device__ __forceinline__ void mitm_run(const MitmArgs& A, u128 s1, u128 M, uint64_t& r0s,
+                                         uint64_t& r0e, uint64_t& r1s, uint64_t& r1e) {
+  const u128 a = es::mk128(A.a_lo, A.a_hi), len = es::mk128(A.len_lo, A.len_hi);
+  const u128 L = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
+  const u128 E = L + len;                          // < 2M
+  r0s = mitm_lower_bound(A, L);
+  r1s = r1e = 0;
+  if (E <= M) {
+    r0e = E == M ? A.n2 : mitm_lower_bound(A, E);
+  } else {
+    r0e = A.n2;
+    r1e = mitm_lower_bound(A, E - M);
+  }
+}
+
+template <bool WAVE>
+__global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, uint64_t count) {
+  const u128 M = es::mk128(A.M_lo, A.M_hi);
+  unsigned long long tested = 0;
+  if (!WAVE) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) {
+      const u128 s1 = mitm_s1(A, i0 + t, M);
+      uint64_t r0s, r0e, r1s, r1e;
+      mitm_run(A, s1, M, r0s, r0e, r1s, r1e);
+      for (int part = 0; part < 2; ++part) {
+        const uint64_t st = part ? r1s : r0s, en = part ? r1e : r0e;
+        for (uint64_t u = st; u < en; ++u) {
+          u128 s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
+          if (s >= M) s -= M;
+          ++tested;
+          if (mitm_test(A, s)) mitm_emit(A, s);
+        }
+      }
+    }
+  } else {
+    // one wave per s1: every lane decodes and searches the same s1 (uniform, broadcast loads),
+    // then the wave walks the run with lane stride 64
+    const uint32_t lane = __lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < count;
+         t += waves) {
+      const u128 s1 = mitm_s1(A, i0 + t, M);
+      uint64_t r0s, r0e, r1s, r1e;
+      mitm_run(A, s1, M, r0s, r0e, r1s, r1e);
+      for (int part = 0; part < 2; ++part) {
+        const uint64_t st = part ? r1s : r0s, en = part ? r1e : r0e;
+        for (uint64_t u = st + lane; u < en; u += 64) {
+          u128 s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
+          if (s >= M) s -= M;
+          ++tested;
+          if (mitm_test(A, s)) mitm_emit(A, s);
+        }
+      }
+    }
+  }
+  if (tested) atomicAdd(A.tested, tested);
+}
+
+// table[i] = CRT partial sum of half-2 index i (mixed radix, ring 0 fastest)
+__global__ void __launch_bounds__(256)
+es_mitm_table_kernel(uint32_t w2, const uint32_t* __restrict__ radix2,
+                     const uint32_t* __restrict__ off2, const uint64_t* __restrict__ t2lo,
+                     const uint64_t* __restrict__ t2hi, uint64_t M_lo, uint64_t M_hi, uint64_t n2,
+                     uint64_t* __restrict__ hi_out, uint64_t* __restrict__ lo_out) {
+  const u128 M = es::mk128(M_lo, M_hi);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    u128 s = 0;
+    uint64_t rem = i;
+    for (uint32_t j = 0; j < w2; ++j) {
+      const uint32_t r = radix2[j];
+      const uint32_t d = (uint32_t)(rem % r);
+      rem /= r;
+      const uint32_t o = off2[j] + d;
+      s += es::mk128(t2lo[o], t2hi[o]);
+      if (s >= M) s -= M;
+    }
+    hi_out[i] = (uint64_t)(s >> 64);
+    lo_out[i] = (uint64_t)s;
+  }
+}
+
+}  // namespace
+
+struct es_mitm {
+  hipStream_t stream = nullptr;
+  u128 M = 0;
+  uint64_t n1 = 1, n2 = 1;
+  uint32_t w1 = 0, nf = 0, wave = 0;
+  std::vector<uint32_t> filt_q, frec;
+  std::vector<void*> allocs;
+  uint32_t *d_radix1 = nullptr, *d_off1 = nullptr, *d_frec = nullptr, *d_fbits = nullptr;
+  uint64_t *d_t1lo = nullptr, *d_t1hi = nullptr, *d_s2hi = nullptr, *d_s2lo = nullptr;
+  unsigned long long *d_out = nullptr, *d_cnt = nullptr;  // d_cnt[0] survivors, [1] tested
+  uint64_t out_cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  uint64_t last_tested = 0;
+  int cus = 256;
+  ~es_mitm() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : allocs) hipFree(p);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    ES_HIP(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs.push_back(q);
+    *p = (T*)q;
+    return ES_OK;
+  }
+};
+
+extern "C" {
+
+int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
+  if (!d || !out) return fail(ES_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (d->abi_version != ES_ABI_VERSION) return fail(ES_ERR_INVALID, "ABI version mismatch");
+  const u128 M = es::mk128(d->M_lo, d->M_hi);
+  if (M == 0 || (M >> 126) != 0) return fail(ES_ERR_INVALID, "wheel modulus must be in [1, 2^126)");
+  if (d->n1 < 1 || d->n2 < 1 || !d->radix1 || !d->radix2 || !d->term1_lo || !d->term1_hi ||
+      !d->term2_lo || !d->term2_hi)
+    return fail(ES_ERR_INVALID, "each half needs at least one ring (a unit ring for an empty half)");
+  std::unique_ptr<es_mitm> g(new es_mitm());
+  es_mitm* m = g.get();
+  m->M = M;
+  m->w1 = d->n1;
+  if (d->walk > 2) return fail(ES_ERR_INVALID, "walk must be 0, 1 or 2");
+  m->wave = d->walk;
+  // half 1: radices, offsets, terms
+  std::vector<uint32_t> off1(d->n1);
+  uint64_t t1 = 0;
+  for (uint32_t j = 0; j < d->n1; ++j) {
+    if (d->radix1[j] == 0) return fail(ES_ERR_INVALID, "empty ring");
+    off1[j] = (uint32_t)t1;
+    t1 += d->radix1[j];
+    m->n1 *= d->radix1[j];
+    if (m->n1 >= (1ull << 62)) return fail(ES_ERR_LIMIT, "half 1 above 2^62 entries");
+  }
+  std::vector<uint32_t> off2(d->n2);
+  uint64_t t2 = 0;
+  for (uint32_t j = 0; j < d->n2; ++j) {
+    if (d->radix2[j] == 0) return fail(ES_ERR_INVALID, "empty ring");
+    off2[j] = (uint32_t)t2;
+    t2 += d->radix2[j];
+    m->n2 *= d->radix2[j];
+    if (m->n2 >= (1ull << 32)) return fail(ES_ERR_LIMIT, "half 2 (the table) above 2^32 entries");
+  }
+  for (uint64_t i = 0; i < t1; ++i)
+    if (es::mk128(d->term1_lo[i], d->term1_hi[i]) >= M) return fail(ES_ERR_INVALID, "term not reduced mod M");
+  for (uint64_t i = 0; i < t2; ++i)
+    if (es::mk128(d->term2_lo[i], d->term2_hi[i]) >= M) return fail(ES_ERR_INVALID, "term not reduced mod M");
+  // filters
+  m->nf = d->n_filters;
+  if (m->nf && (!d->filt_q || !d->filt_off || !d->filt_bits)) return fail(ES_ERR_INVALID, "filter arrays missing");
+  m->frec.assign((size_t)kFiltRec * std::max<uint32_t>(m->nf, 1), 0);
+  for (uint32_t f = 0; f < m->nf; ++f) {
+    const uint32_t q = d->filt_q[f];
+    if (q < 2 || q >= (1u << 24)) return fail(ES_ERR_LIMIT, "filter modulus outside [2, 2^24)");
+    if ((uint64_t)d->filt_off[f] + (q + 31) / 32 > d->n_filt_words)
+      return fail(ES_ERR_INVALID, "filter bitmap outside filt_bits");
+    uint32_t* r = &m->frec[(size_t)kFiltRec * f];
+    r[0] = q;
+    r[1] = d->filt_off[f];
+    r[2] = mod_u128((u128)1 << 32, q);
+    r[3] = mod_u128((u128)1 << 64, q);
+    r[4] = mod_u128((u128)1 << 96, q);
+    r[5] = 0;  // t*M mod q, per scan
+    m->filt_q.push_back(q);
+  }
+  m->out_cap = d->out_cap ? d->out_cap : (1ull << 20);
+  ES_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  ES_HIP(hipEventCreate(&m->ev0));
+  ES_HIP(hipEventCreate(&m->ev1));
+  int dev = 0;
+  ES_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  ES_HIP(hipGetDeviceProperties(&prop, dev));
+  m->cus = prop.multiProcessorCount;
+  int rc;
+  uint32_t *d_radix2 = nullptr, *d_off2 = nullptr;
+  uint64_t *d_t2lo = nullptr, *d_t2hi = nullptr;
+  if ((rc = m->alloc(&m->d_radix1, d->n1)) || (rc = m->alloc(&m->d_off1, d->n1)) ||
+      (rc = m->alloc(&m->d_t1lo, t1)) || (rc = m->alloc(&m->d_t1hi, t1)) ||
+      (rc = m->alloc(&d_radix2, d->n2)) || (rc = m->alloc(&d_off2, d->n2)) ||
+      (rc = m->alloc(&d_t2lo, t2)) || (rc = m->alloc(&d_t2hi, t2)) ||
+      (rc = m->alloc(&m->d_frec, m->frec.size())) ||
+      (rc = m->alloc(&m->d_fbits, std::max<uint32_t>(d->n_filt_words, 1))) ||
+      (rc = m->alloc(&m->d_s2hi, m->n2)) || (rc = m->alloc(&m->d_s2lo, m->n2)) ||
+      (rc = m->alloc(&m->d_out, 2 * m->out_cap)) || (rc = m->alloc(&m->d_cnt, 2)))
+    return rc;
+  ES_HIP(hipMemcpy(m->d_radix1, d->radix1, d->n1 * 4, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(m->d_off1, off1.data(), d->n1 * 4, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(m->d_t1lo, d->term1_lo, t1 * 8, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(m->d_t1hi, d->term1_hi, t1 * 8, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(d_radix2, d->radix2, d->n2 * 4, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(d_off2, off2.data(), d->n2 * 4, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(d_t2lo, d->term2_lo, t2 * 8, hipMemcpyHostToDevice));
+  ES_HIP(hipMemcpy(d_t2hi, d->term2_hi, t2 * 8, hipMemcpyHostToDevice));
+  if (m->nf)
+    ES_HIP(hipMemcpy(m->d_fbits, d->filt_bits, (size_t)d->n_filt_words * 4, hipMemcpyHostToDevice));
+  // the half-2 table: decode, then sort by low word and (stably) by high word
+  uint64_t *k_hi = nullptr, *k_lo = nullptr;
+  if ((rc = m->alloc(&k_hi, m->n2)) || (rc = m->alloc(&k_lo, m->n2))) return rc;
+  const uint64_t blocks = std::min<uint64_t>((m->n2 + 255) / 256, (uint64_t)m->cus * 64);
+  hipLaunchKernelGGL(es_mitm_table_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
+                     d->n2, d_radix2, d_off2, d_t2lo, d_t2hi, d->M_lo, d->M_hi, m->n2, k_hi, k_lo);
+  ES_HIP(hipGetLastError());
+  size_t tmp_bytes = 0;
+  ES_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, k_lo, m->d_s2lo, k_hi, m->d_s2hi,
+                                   (uint32_t)m->n2, 0, 64, m->stream));
+  void* tmp = nullptr;
+  ES_HIP(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
+  m->allocs.push_back(tmp);
+  ES_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, k_lo, m->d_s2lo, k_hi, m->d_s2hi,
+                                   (uint32_t)m->n2, 0, 64, m->stream));
+  // second pass by the high word; LSD radix sort is stable, so equal highs keep low order
+  ES_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, m->d_s2hi, k_hi, m->d_s2lo, k_lo,
+                                   (uint32_t)m->n2, 0, 64, m->stream));
+  ES_HIP(hipMemcpyAsync(m->d_s2hi, k_hi, m->n2 * 8, hipMemcpyDeviceToDevice, m->stream));
+  ES_HIP(hipMemcpyAsync(m->d_s2lo, k_lo, m->n2 * 8, hipMemcpyDeviceToDevice, m->stream));
+  ES_HIP(hipStreamSynchronize(m->stream));
+  *out = g.release();
+  return ES_OK;
+}
+
+int es_mitm_destroy(es_mitm* m) {
+  delete m;
+  return ES_OK;
+}
+
+int es_mitm_sizes(es_mitm* m, uint64_t* n1, uint64_t* n2) {
+  if (!m) return fail(ES_ERR_INVALID, "null argument");
+  if (n1) *n1 = m->n1;
+  if (n2) *n2 = m->n2;
+  return ES_OK;
+}
+
+int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t b_lo,
+                 uint64_t b_hi, uint64_t* count) {
+  if (!m) return fail(ES_ERR_INVALID, "null argument");
+  const u128 a = es::mk128(a_lo, a_hi), b = es::mk128(b_lo, b_hi);
+  if (!(a < b) || b > m->M) return fail(ES_ERR_INVALID, "window must satisfy 0 <= a < b <= M");
+  const u128 len = b - a;
+  // block constants t*M mod q
+  for (uint32_t f = 0; f < m->nf; ++f) {
+    const uint32_t q = m->filt_q[f];
+    m->frec[(size_t)kFiltRec * f + 5] = (uint32_t)(((uint64_t)(t % q) * mod_u128(m->M, q)) % q);
+  }
+  ES_HIP(hipMemcpyAsync(m->d_frec, m->frec.data(), m->frec.size() * 4, hipMemcpyHostToDevice,
+                        m->stream));
+  ES_HIP(hipMemsetAsync(m->d_cnt, 0, 16, m->stream));
+  MitmArgs A{};
+  A.M_lo = (uint64_t)m->M;
+  A.M_hi = (uint64_t)(m->M >> 64);
+  A.a_lo = a_lo;
+  A.a_hi = a_hi;
+  A.len_lo = (uint64_t)len;
+  A.len_hi = (uint64_t)(len >> 64);
+  A.w1 = m->w1;
+  A.nf = m->nf;
+  A.radix1 = m->d_radix1;
+  A.off1 = m->d_off1;
+  A.t1lo = m->d_t1lo;
+  A.t1hi = m->d_t1hi;
+  A.n2 = m->n2;
+  A.s2hi = m->d_s2hi;
+  A.s2lo = m->d_s2lo;
+  A.frec = m->d_frec;
+  A.fbits = m->d_fbits;
+  A.out = m->d_out;
+  A.out_n = m->d_cnt;
+  A.tested = m->d_cnt + 1;
+  A.out_cap = m->out_cap;
+  // expected run length n2 * len / M decides the walk when not fixed, and bounds each launch
+  // to ~2^34 candidates (long scans become many launches of bounded length)
+  const long double run = (long double)m->n2 * (long double)len / (long double)m->M;
+  bool wave = m->wave == 2 || (m->wave == 0 && run >= 8.0L);
+  const long double per = (long double)(1ull << 34) / (run > 1.0L ? run : 1.0L);
+  const uint64_t chunk = (uint64_t)std::min<long double>(std::max<long double>(per, 65536.0L),
+                                                         (long double)(1ull << 26));
+  ES_HIP(hipEventRecord(m->ev0, m->stream));
+  for (uint64_t s = 0; s < m->n1; s += chunk) {
+    const uint64_t cnt = std::min<uint64_t>(chunk, m->n1 - s);
+    const uint64_t per_block = wave ? 4 : 256;
+    const uint64_t blocks =
+        std::max<uint64_t>(1, std::min<uint64_t>((cnt + per_block - 1) / per_block, (uint64_t)m->cus * 64));
+    if (wave)
+      hipLaunchKernelGGL(es_mitm_kernel<true>, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
+                         A, s, cnt);
+    else
+      hipLaunchKernelGGL(es_mitm_kernel<false>, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
+                         A, s, cnt);
+    ES_HIP(hipGetLastError());
+  }
+  ES_HIP(hipEventRecord(m->ev1, m->stream));
+  unsigned long long c[2] = {0, 0};
+  ES_HIP(hipMemcpyAsync(c, m->d_cnt, 16, hipMemcpyDeviceToHost, m->stream));
+  ES_HIP(hipStreamSynchronize(m->stream));
+  float ms = 0.f;
+  ES_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+  m->last_ms = ms;
+  m->last_tested = c[1];
+  if (count) *count = c[0];
+  if (c[0] > m->out_cap) return fail(ES_ERR_OVERFLOW, "survivor buffer overflow");
+  return ES_OK;
+}
+
+int es_mitm_read(es_mitm* m, uint64_t* s_lohi, uint64_t count) {
+  if (!m || (count && !s_lohi)) return fail(ES_ERR_INVALID, "null argument");
+  if (count > m->out_cap) return fail(ES_ERR_INVALID, "read past the survivor buffer");
+  if (count) ES_HIP(hipMemcpy(s_lohi, m->d_out, count * 16, hipMemcpyDeviceToHost));
+  return ES_OK;
+}
+
+int es_mitm_last(es_mitm* m, double* ms, uint64_t* candidates) {
+  if (!m) return fail(ES_ERR_INVALID, "null argument");
+  if (ms) *ms = m->last_ms;
+  if (candidates) *candidates = m->last_tested;
+  return ES_OK;
+}
+
+}  // extern "C"

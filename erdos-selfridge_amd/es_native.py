@@ -93,6 +93,13 @@ def lib() -> C.CDLL:
         "es_fpass_read_stream": (C.c_int, [eng, C.c_uint64, C.c_uint64, u64p, u32p]),
         "es_fpass_last_ms": (C.c_int, [eng, C.POINTER(C.c_double)]),
         "es_fpass_set_shape": (C.c_int, [eng, C.c_uint32, C.c_uint32]),
+        "es_mitm_create": (C.c_int, [C.c_void_p, C.POINTER(eng)]),
+        "es_mitm_destroy": (C.c_int, [eng]),
+        "es_mitm_sizes": (C.c_int, [eng, u64p, u64p]),
+        "es_mitm_scan": (C.c_int, [eng, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                   C.c_uint64, u64p]),
+        "es_mitm_read": (C.c_int, [eng, u64p, C.c_uint64]),
+        "es_mitm_last": (C.c_int, [eng, C.POINTER(C.c_double), u64p]),
         "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
         "es_mlp_destroy": (C.c_int, [eng]),
         "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),

@@ -253,3 +253,79 @@ int64_t eso_filter_values(uint32_t k, uint32_t nf, const uint32_t* primes,
   }
   return (int64_t)n;
 }
+
+/* Every n in [n_lo, n_hi) that dominates k in base p for every prime p <= k (the Kummer test of
+ * ref/erdos_ref.py:40-55 without the n > k+1 condition), ascending: the exact survivor set of an
+ * n-space window whatever wheel enumerates it.  pthreads over equal slices.  Returns the count
+ * (only cap values are written, lo/hi u64 pairs), or -1 on bad arguments. */
+typedef struct {
+  uint32_t k, np;
+  const uint32_t* primes;
+  u128 lo, hi;
+  u128* out;
+  uint64_t cap, n;
+} eso_range_job;
+
+static void* eso_range_worker(void* arg) {
+  eso_range_job* J = (eso_range_job*)arg;
+  for (u128 n = J->lo; n < J->hi; ++n) {
+    int ok = 1;
+    for (uint32_t i = 0; i < J->np && ok; ++i) {
+      const uint32_t p = J->primes[i];
+      uint32_t kk = J->k;
+      u128 m = n;
+      while (kk) {
+        if (kk % p > (uint32_t)(m % p)) { ok = 0; break; }
+        kk /= p;
+        m /= p;
+      }
+    }
+    if (ok) {
+      if (J->n < J->cap) J->out[J->n] = n;
+      J->n++;
+    }
+  }
+  return NULL;
+}
+
+int64_t eso_good_range(uint32_t k, uint64_t lo_lo, uint64_t lo_hi, uint64_t hi_lo, uint64_t hi_hi,
+                       uint32_t nthreads, uint64_t* out_lohi, uint64_t cap) {
+  const u128 lo = ((u128)lo_hi << 64) | lo_lo, hi = ((u128)hi_hi << 64) | hi_lo;
+  if (hi < lo || k < 2) return -1;
+  uint32_t primes[ESO_MAX_FILT], np = 0;
+  for (uint32_t p = 2; p <= k && np < ESO_MAX_FILT; ++p) {
+    int pr = 1;
+    for (uint32_t d = 2; d * d <= p; ++d)
+      if (p % d == 0) { pr = 0; break; }
+    if (pr) primes[np++] = p;
+  }
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  const u128 span = hi - lo;
+  if (span < (u128)nthreads * 4096) nthreads = 1;
+  eso_range_job* jobs = (eso_range_job*)calloc(nthreads, sizeof(eso_range_job));
+  pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+  for (uint32_t i = 0; i < nthreads; ++i) {
+    jobs[i].k = k;
+    jobs[i].np = np;
+    jobs[i].primes = primes;
+    jobs[i].lo = lo + span * i / nthreads;
+    jobs[i].hi = lo + span * (i + 1) / nthreads;
+    jobs[i].cap = cap;
+    jobs[i].out = (u128*)malloc(sizeof(u128) * (cap ? cap : 1));
+    pthread_create(&th[i], NULL, eso_range_worker, &jobs[i]);
+  }
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nthreads; ++i) {
+    pthread_join(th[i], NULL);
+    for (uint64_t j = 0; j < jobs[i].n && j < cap && total + j < cap; ++j) {
+      out_lohi[2 * (total + j)] = (uint64_t)jobs[i].out[j];
+      out_lohi[2 * (total + j) + 1] = (uint64_t)(jobs[i].out[j] >> 64);
+    }
+    total += jobs[i].n;
+    free(jobs[i].out);
+  }
+  free(jobs);
+  free(th);
+  return (int64_t)total;
+}

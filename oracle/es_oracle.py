@@ -196,6 +196,9 @@ def lib():
         L.eso_filter_values.restype = C.c_int64
         L.eso_filter_values.argtypes = [C.c_uint32, C.c_uint32, u32p, u64p, C.c_uint64, u32p,
                                         C.c_uint64, u32p, C.c_uint64]
+        L.eso_good_range.restype = C.c_int64
+        L.eso_good_range.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                     C.c_uint32, u64p, C.c_uint64]
         _lib = L
     return _lib
 
@@ -265,3 +268,18 @@ def c_filter_values(k: int, primes, values, ids, cap: int = 1 << 22):
     if n < 0:
         raise ValueError("oracle rejected the input")
     return out[:min(n, cap)].copy()
+
+
+def c_good_range(k: int, n_lo: int, n_hi: int, threads: int = 8, cap: int = 1 << 20):
+    """(count, ascending list) of every n in [n_lo, n_hi) dominating k for all p <= k
+    (eso_good_range: brute force, independent of any wheel)."""
+    import numpy as np
+    L = lib()
+    out = np.zeros(2 * max(1, cap), dtype=np.uint64)
+    M64 = (1 << 64) - 1
+    n = L.eso_good_range(k, n_lo & M64, n_lo >> 64, n_hi & M64, n_hi >> 64, threads,
+                         out.ctypes.data_as(C.POINTER(C.c_uint64)), cap)
+    if n < 0:
+        raise ValueError("oracle rejected the range")
+    m = min(n, cap)
+    return n, [(int(out[2 * i + 1]) << 64) | int(out[2 * i]) for i in range(m)]

@@ -44,15 +44,18 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="batch, k200: autotune the launch shape of each wheel first")
     ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch", "k200",
-                                           "surrogate"),
+                                           "surrogate", "mitm"),
                     default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
                          "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters); "
                          "batch: k=300..320 batched launches; k200: full-power wheel window; "
-                         "surrogate: learned planner vs plain planner, k=350..400")
+                         "surrogate: learned planner vs plain planner, k=350..400; "
+                         "mitm: bounded n-window search for g(k)")
     ap.add_argument("--fpass-log2", type=int, default=30)
     ap.add_argument("--k-step", type=int, default=1, help="surrogate: every k-step-th k")
     ap.add_argument("--candidates", type=int, default=100_000, help="surrogate: plans per k")
+    ap.add_argument("--mitm-k", type=int, default=180)
+    ap.add_argument("--mitm-mem", type=float, default=1.0, help="mitm: table budget, GB")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -68,6 +71,8 @@ def main() -> int:
         return bench_k200(a)
     if a.workload == "surrogate":
         return bench_surrogate(a)
+    if a.workload == "mitm":
+        return bench_mitm(a)
     if a.workload == "copy":
         import torch
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")       # 1 GiB
@@ -498,6 +503,69 @@ def bench_surrogate(a) -> int:
                    "plain_s": r["plain"]["time_to_first_block_s"],
                    "surrogate_s": r["surrogate"]["time_to_first_block_s"]} for r in rows]}),
         flush=True)
+    return 0
+
+
+def bench_mitm(a) -> int:
+    """Meet-in-the-middle search for g(k): windows [0, B) with B = 3E, quadrupled until a good
+    n > k+1 appears (the b-file value must be returned).  Reports candidates tested per second
+    and the n-range covered per second (device time), the HBM rate of the table reads, and the
+    CPU oracle's brute-force range scan (Kummer test of every n) on a bounded n-window."""
+    import math
+
+    import es_mitm as MM
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import es_oracle as O
+    bf = {}
+    for line in open(os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt")):
+        parts = line.split()
+        if len(parts) == 2:
+            bf[int(parts[0])] = int(parts[1])
+    k = a.mitm_k
+    t0 = time.perf_counter()
+    s = MM.MitmSieve(k, mem_gb=a.mitm_mem)
+    build_s = time.perf_counter() - t0
+    for _ in range(max(0, a.warmup)):
+        MM.mitm_search(k, sieve=s)
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(max(1, a.steps)):
+        runs.append(MM.mitm_search(k, sieve=s))
+    wall = (time.perf_counter() - t0) / max(1, a.steps)
+    r = runs[-1]
+    assert r["status"] == "FOUND" and r["g"] == bf[k], r
+    dev_ms = min(x["device_ms"] for x in runs)
+    cand = r["candidates"]
+    n_range = r["bound"]
+    n1, n2 = s.sizes
+    bytes_alg = cand * 16 + n1 * 2 * max(1, math.ceil(math.log2(n2))) * 16
+    achieved = bytes_alg / (dev_ms / 1e3)
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        span = 1 << 27
+        lo = r["g"] - span // 2
+        c0 = time.perf_counter()
+        _, got = O.c_good_range(k, lo, lo + span, threads=threads)
+        cdt = time.perf_counter() - c0
+        assert r["g"] in got
+        cpu = {"value": span / cdt, "unit": "n/s", "cores": threads, "kind": "port",
+               "sample": f"Kummer test of every n in a 2^27-wide range around g({k}) "
+                         f"({cdt:.1f} s); contains g"}
+    print(json.dumps({
+        "metric": f"meet-in-the-middle search for g({k}): n-range covered per second",
+        "value": n_range / (dev_ms / 1e3), "unit": "n/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u128/u32 integer",
+        "data": "exact search; g(k) checked against the b-file",
+        "config": {"workload": f"k={k}, table budget {a.mitm_mem} GB", "wheel": s.wheel,
+                   "M_bits": s.M.bit_length(), "S1": n1, "S2": n2, "bound": str(n_range),
+                   "g": str(r["g"]), "build_s": build_s},
+        "candidates": cand, "candidates_per_s": cand / (dev_ms / 1e3), "device_ms": dev_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes": "16 B per candidate + 2 binary searches x ceil(log2 S2) x 16 B per s1"},
+        "cpu_baseline": cpu}), flush=True)
     return 0
 
 

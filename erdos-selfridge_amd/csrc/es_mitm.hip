@@ -39,6 +39,8 @@ struct MitmArgs {
   const uint64_t* s2hi;
   const uint64_t* s2lo;
   const uint32_t* frec;
+  const double* fdrec;           // per filter: 2^32, 2^64, 2^96 mod q, q, 1/q, t*M mod q
+  uint32_t n_first;              // filters every candidate is tested against before packing
   const uint32_t* fbits;
   unsigned long long* out;       // survivors: lo, hi pairs
   unsigned long long* out_n;
@@ -90,6 +92,31 @@ __device__ __forceinline__ bool mitm_test(const MitmArgs& A, u128 s) {
   return true;
 }
 
+// Filters [j0, j1) in FP64 (every q < 2^18: limb * (2^32k mod q) < 2^50, the sum of the four
+// terms and t*M mod q stays below 2^52 and is exact; the quotient from one multiply by 1/q is
+// corrected by one q either way).  EXIT 0: every filter (dense lanes); 1: the loop ends once no
+// lane of the wave is alive; 2: each lane stops at its first failing filter.
+template <int EXIT>
+__device__ __forceinline__ bool mitm_test_fp(const MitmArgs& A, u128 s, bool alive, uint32_t j0,
+                                             uint32_t j1) {
+  const double d0 = (double)(uint32_t)s, d1 = (double)(uint32_t)(s >> 32),
+               d2 = (double)(uint32_t)(s >> 64), d3 = (double)(uint32_t)(s >> 96);
+  for (uint32_t j = j0; j < j1; ++j) {
+    if (EXIT == 1 && !__any(alive)) break;
+    if (EXIT == 2 && !alive) break;
+    const double* f = A.fdrec + 8 * j;
+    const double acc = fma(d3, f[2], fma(d2, f[1], fma(d1, f[0], d0 + f[5])));
+    const double q = f[3];
+    double r = fma(-floor(acc * f[4]), q, acc);
+    r = r < 0.0 ? r + q : r;
+    r = r >= q ? r - q : r;
+    const uint32_t ri = (uint32_t)r;
+    const uint32_t off = A.frec[kFiltRec * j + 1];
+    alive = alive && ((A.fbits[off + (ri >> 5)] >> (ri & 31u)) & 1u);
+  }
+  return alive;
+}
+
 __device__ __forceinline__ void mitm_emit(const MitmArgs& A, u128 s) {
   const unsigned long long slot = atomicAdd(A.out_n, 1ull);
   if (slot < A.out_cap) {
@@ -130,14 +157,34 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
           u128 s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
           if (s >= M) s -= M;
           ++tested;
-          if (mitm_test(A, s)) mitm_emit(A, s);
+          if (A.fdrec ? mitm_test_fp<2>(A, s, true, 0, A.nf) : mitm_test(A, s))
+            mitm_emit(A, s);
         }
       }
     }
   } else {
-    // one wave per s1: every lane decodes and searches the same s1 (uniform, broadcast loads),
-    // then the wave walks the run with lane stride 64
+    // One wave per s1: every lane decodes and searches the same s1 (uniform, broadcast loads),
+    // then the wave walks the run with lane stride 64.  With FP64 filters: two stages -- every
+    // candidate against the first n_first filters, the few left packed into the wave's LDS
+    // list, the remaining filters run over the full list (lanes dense, early exit per wave).
+    __shared__ uint4 q_all[4 * 64];  // 256 threads = 4 waves x 64 entries (s as 4 limbs)
     const uint32_t lane = __lane_id();
+    uint4* qv = q_all + (threadIdx.x >> 6) * 64u;
+    const bool fp = A.fdrec != nullptr;
+    uint32_t cnt = 0;  // wave-uniform
+    auto flush = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const bool have = lane < cnt;
+      const uint4 v = have ? qv[lane] : make_uint4(0, 0, 0, 0);
+      const u128 s = es::mk128(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+      if (mitm_test_fp<1>(A, s, have, A.n_first, A.nf)) mitm_emit(A, s);
+      cnt = 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < count;
          t += waves) {
@@ -146,14 +193,33 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
       mitm_run(A, s1, M, r0s, r0e, r1s, r1e);
       for (int part = 0; part < 2; ++part) {
         const uint64_t st = part ? r1s : r0s, en = part ? r1e : r0e;
-        for (uint64_t u = st + lane; u < en; u += 64) {
-          u128 s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
-          if (s >= M) s -= M;
-          ++tested;
-          if (mitm_test(A, s)) mitm_emit(A, s);
+        for (uint64_t u0 = st; u0 < en; u0 += 64) {  // wave-uniform bound
+          const uint64_t u = u0 + lane;
+          const bool ok = u < en;
+          u128 s = 0;
+          if (ok) {
+            s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
+            if (s >= M) s -= M;
+            ++tested;
+          }
+          if (!fp) {
+            if (ok && mitm_test(A, s)) mitm_emit(A, s);
+            continue;
+          }
+          const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
+          const unsigned long long mask = __ballot(alive);
+          const uint32_t n = __popcll(mask);
+          if (cnt + n > 64u) flush();
+          if (alive) {
+            const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+            qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
+                                  (uint32_t)(s >> 96));
+          }
+          cnt += n;
         }
       }
     }
+    if (fp && cnt) flush();
   }
   if (tested) atomicAdd(A.tested, tested);
 }
@@ -190,6 +256,9 @@ struct es_mitm {
   uint64_t n1 = 1, n2 = 1;
   uint32_t w1 = 0, nf = 0, wave = 0;
   std::vector<uint32_t> filt_q, frec;
+  std::vector<double> fdrec;  // FP64 filter records when every q < 2^18
+  double* d_fdrec = nullptr;
+  uint32_t n_first = 2;
   std::vector<void*> allocs;
   uint32_t *d_radix1 = nullptr, *d_off1 = nullptr, *d_frec = nullptr, *d_fbits = nullptr;
   uint64_t *d_t1lo = nullptr, *d_t1hi = nullptr, *d_s2hi = nullptr, *d_s2lo = nullptr;
@@ -274,6 +343,22 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
     r[5] = 0;  // t*M mod q, per scan
     m->filt_q.push_back(q);
   }
+  bool fp_ok = m->nf > 0;
+  for (uint32_t f = 0; f < m->nf; ++f) fp_ok = fp_ok && d->filt_q[f] < (1u << 18);
+  if (fp_ok) {
+    m->fdrec.assign((size_t)8 * m->nf, 0.0);
+    for (uint32_t f = 0; f < m->nf; ++f) {
+      const uint32_t* r = &m->frec[(size_t)kFiltRec * f];
+      double* x = &m->fdrec[(size_t)8 * f];
+      x[0] = r[2];
+      x[1] = r[3];
+      x[2] = r[4];
+      x[3] = (double)r[0];
+      x[4] = 1.0 / (double)r[0];
+      x[5] = 0.0;  // t*M mod q, per scan
+    }
+  }
+  m->n_first = std::min<uint32_t>(2, m->nf);
   m->out_cap = d->out_cap ? d->out_cap : (1ull << 20);
   ES_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   ES_HIP(hipEventCreate(&m->ev0));
@@ -291,6 +376,7 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
       (rc = m->alloc(&d_radix2, d->n2)) || (rc = m->alloc(&d_off2, d->n2)) ||
       (rc = m->alloc(&d_t2lo, t2)) || (rc = m->alloc(&d_t2hi, t2)) ||
       (rc = m->alloc(&m->d_frec, m->frec.size())) ||
+      (rc = m->alloc(&m->d_fdrec, std::max<size_t>(m->fdrec.size(), 1))) ||
       (rc = m->alloc(&m->d_fbits, std::max<uint32_t>(d->n_filt_words, 1))) ||
       (rc = m->alloc(&m->d_s2hi, m->n2)) || (rc = m->alloc(&m->d_s2lo, m->n2)) ||
       (rc = m->alloc(&m->d_out, 2 * m->out_cap)) || (rc = m->alloc(&m->d_cnt, 2)))
@@ -352,9 +438,13 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
   for (uint32_t f = 0; f < m->nf; ++f) {
     const uint32_t q = m->filt_q[f];
     m->frec[(size_t)kFiltRec * f + 5] = (uint32_t)(((uint64_t)(t % q) * mod_u128(m->M, q)) % q);
+    if (!m->fdrec.empty()) m->fdrec[(size_t)8 * f + 5] = (double)m->frec[(size_t)kFiltRec * f + 5];
   }
   ES_HIP(hipMemcpyAsync(m->d_frec, m->frec.data(), m->frec.size() * 4, hipMemcpyHostToDevice,
                         m->stream));
+  if (!m->fdrec.empty())
+    ES_HIP(hipMemcpyAsync(m->d_fdrec, m->fdrec.data(), m->fdrec.size() * 8,
+                          hipMemcpyHostToDevice, m->stream));
   ES_HIP(hipMemsetAsync(m->d_cnt, 0, 16, m->stream));
   MitmArgs A{};
   A.M_lo = (uint64_t)m->M;
@@ -373,6 +463,8 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
   A.s2hi = m->d_s2hi;
   A.s2lo = m->d_s2lo;
   A.frec = m->d_frec;
+  A.fdrec = m->fdrec.empty() ? nullptr : m->d_fdrec;
+  A.n_first = m->n_first;
   A.fbits = m->d_fbits;
   A.out = m->d_out;
   A.out_n = m->d_cnt;

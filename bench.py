@@ -54,8 +54,8 @@ def main() -> int:
     ap.add_argument("--fpass-log2", type=int, default=30)
     ap.add_argument("--k-step", type=int, default=1, help="surrogate: every k-step-th k")
     ap.add_argument("--candidates", type=int, default=100_000, help="surrogate: plans per k")
-    ap.add_argument("--mitm-k", type=int, default=180)
-    ap.add_argument("--mitm-mem", type=float, default=1.0, help="mitm: table budget, GB")
+    ap.add_argument("--mitm-k", type=int, default=200)
+    ap.add_argument("--mitm-mem", type=float, default=64.0, help="mitm: table budget, GB")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -224,6 +224,104 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
   if (tested) atomicAdd(A.tested, tested);
 }
 
+// Sorted sweep: half 1 is a sorted table too.  A wave takes `chunk` consecutive s1 of it; as
+// s1 grows, (a - s1) mod M and (a + len - s1) mod M fall, so the two run ends move down the
+// half-2 table by short linear steps; a binary search is needed only at the chunk's first s1
+// and where an end wraps past 0 or M.  Runs are walked by the wave as in es_mitm_kernel<true>.
+__global__ void __launch_bounds__(256)
+es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
+                     const uint64_t* __restrict__ s1lo, uint64_t n1, uint64_t chunk,
+                     uint64_t c_lo, uint64_t c_hi) {
+  const u128 M = es::mk128(A.M_lo, A.M_hi);
+  const u128 a = es::mk128(A.a_lo, A.a_hi), len = es::mk128(A.len_lo, A.len_hi);
+  __shared__ uint4 q_all[4 * 64];
+  const uint32_t lane = __lane_id();
+  uint4* qv = q_all + (threadIdx.x >> 6) * 64u;
+  const bool fp = A.fdrec != nullptr;
+  unsigned long long tested = 0;
+  uint32_t cnt = 0;  // wave-uniform
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool have = lane < cnt;
+    const uint4 v = have ? qv[lane] : make_uint4(0, 0, 0, 0);
+    const u128 s = es::mk128(((uint64_t)v.y << 32) | v.x, ((uint64_t)v.w << 32) | v.z);
+    if (mitm_test_fp<1>(A, s, have, A.n_first, A.nf)) mitm_emit(A, s);
+    cnt = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto walk = [&](u128 s1, uint64_t st, uint64_t en) {
+    for (uint64_t u0 = st; u0 < en; u0 += 64) {  // wave-uniform bound
+      const uint64_t u = u0 + lane;
+      const bool ok = u < en;
+      u128 s = 0;
+      if (ok) {
+        s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
+        if (s >= M) s -= M;
+        ++tested;
+      }
+      if (!fp) {
+        if (ok && mitm_test(A, s)) mitm_emit(A, s);
+        continue;
+      }
+      const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
+      const unsigned long long mask = __ballot(alive);
+      const uint32_t n = __popcll(mask);
+      if (cnt + n > 64u) flush();
+      if (alive) {
+        const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+        qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
+                              (uint32_t)(s >> 96));
+      }
+      cnt += n;
+    }
+  };
+  auto ge = [&](uint64_t u, u128 x) {  // table[u] >= x
+    const uint64_t h = A.s2hi[u], l = A.s2lo[u];
+    const uint64_t xh = (uint64_t)(x >> 64), xl = (uint64_t)x;
+    return h > xh || (h == xh && l >= xl);
+  };
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t c = c_lo + (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       c < c_hi && c * chunk < n1; c += waves) {
+    const uint64_t i_lo = c * chunk, i_hi = n1 - i_lo < chunk ? n1 : i_lo + chunk;
+    uint64_t pX = 0, pY = 0;
+    u128 Xp = 0, Tp = 0;
+    bool wp = false;
+    for (uint64_t i = i_lo; i < i_hi; ++i) {
+      const u128 s1 = es::mk128(s1lo[i], s1hi[i]);
+      const u128 X = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
+      const u128 Y = X + len;                          // < 2M
+      const bool w = Y > M;
+      const u128 T = w ? Y - M : Y;                    // end of the run's last part, <= M
+      if (i == i_lo || X > Xp) {
+        pX = mitm_lower_bound(A, X);
+      } else {
+        while (pX > 0 && ge(pX - 1, X)) --pX;
+      }
+      if (i == i_lo || w != wp || T > Tp) {
+        pY = T == M ? A.n2 : mitm_lower_bound(A, T);
+      } else {
+        while (pY > 0 && ge(pY - 1, T)) --pY;
+      }
+      Xp = X;
+      Tp = T;
+      wp = w;
+      if (!w) {
+        walk(s1, pX, pY);
+      } else {
+        walk(s1, pX, A.n2);
+        walk(s1, 0, pY);
+      }
+    }
+  }
+  if (fp && cnt) flush();
+  if (tested) atomicAdd(A.tested, tested);
+}
+
 // table[i] = CRT partial sum of half-2 index i (mixed radix, ring 0 fastest)
 __global__ void __launch_bounds__(256)
 es_mitm_table_kernel(uint32_t w2, const uint32_t* __restrict__ radix2,
@@ -262,6 +360,7 @@ struct es_mitm {
   std::vector<void*> allocs;
   uint32_t *d_radix1 = nullptr, *d_off1 = nullptr, *d_frec = nullptr, *d_fbits = nullptr;
   uint64_t *d_t1lo = nullptr, *d_t1hi = nullptr, *d_s2hi = nullptr, *d_s2lo = nullptr;
+  uint64_t *d_s1hi = nullptr, *d_s1lo = nullptr;  // sorted half 1 (sweep), or null
   unsigned long long *d_out = nullptr, *d_cnt = nullptr;  // d_cnt[0] survivors, [1] tested
   uint64_t out_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -391,27 +490,65 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
   ES_HIP(hipMemcpy(d_t2hi, d->term2_hi, t2 * 8, hipMemcpyHostToDevice));
   if (m->nf)
     ES_HIP(hipMemcpy(m->d_fbits, d->filt_bits, (size_t)d->n_filt_words * 4, hipMemcpyHostToDevice));
-  // the half-2 table: decode, then sort by low word and (stably) by high word
-  uint64_t *k_hi = nullptr, *k_lo = nullptr;
-  if ((rc = m->alloc(&k_hi, m->n2)) || (rc = m->alloc(&k_lo, m->n2))) return rc;
-  const uint64_t blocks = std::min<uint64_t>((m->n2 + 255) / 256, (uint64_t)m->cus * 64);
-  hipLaunchKernelGGL(es_mitm_table_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
-                     d->n2, d_radix2, d_off2, d_t2lo, d_t2hi, d->M_lo, d->M_hi, m->n2, k_hi, k_lo);
-  ES_HIP(hipGetLastError());
-  size_t tmp_bytes = 0;
-  ES_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, k_lo, m->d_s2lo, k_hi, m->d_s2hi,
-                                   (uint32_t)m->n2, 0, 64, m->stream));
-  void* tmp = nullptr;
-  ES_HIP(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
-  m->allocs.push_back(tmp);
-  ES_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, k_lo, m->d_s2lo, k_hi, m->d_s2hi,
-                                   (uint32_t)m->n2, 0, 64, m->stream));
-  // second pass by the high word; LSD radix sort is stable, so equal highs keep low order
-  ES_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, m->d_s2hi, k_hi, m->d_s2lo, k_lo,
-                                   (uint32_t)m->n2, 0, 64, m->stream));
-  ES_HIP(hipMemcpyAsync(m->d_s2hi, k_hi, m->n2 * 8, hipMemcpyDeviceToDevice, m->stream));
-  ES_HIP(hipMemcpyAsync(m->d_s2lo, k_lo, m->n2 * 8, hipMemcpyDeviceToDevice, m->stream));
-  ES_HIP(hipStreamSynchronize(m->stream));
+  // the half-2 table (and, for the sweep, the half-1 table): decode, then sort by low word and
+  // (stably) by high word; sort scratch is freed once the table is built
+  auto build_table = [&](uint32_t w, const uint32_t* d_radix, const uint32_t* d_off,
+                         const uint64_t* d_tlo, const uint64_t* d_thi, uint64_t n,
+                         uint64_t* out_hi, uint64_t* out_lo) -> int {
+    uint64_t *k_hi = nullptr, *k_lo = nullptr;
+    void* tmp = nullptr;
+    int r = ES_OK;
+    do {
+      if (hipMalloc((void**)&k_hi, n * 8) != hipSuccess || hipMalloc((void**)&k_lo, n * 8) != hipSuccess) {
+        r = fail(ES_ERR_DEVICE, "hipMalloc failed for the sort buffers");
+        break;
+      }
+      const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, (uint64_t)m->cus * 64);
+      hipLaunchKernelGGL(es_mitm_table_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
+                         w, d_radix, d_off, d_tlo, d_thi, d->M_lo, d->M_hi, n, k_hi, k_lo);
+      if (hipGetLastError() != hipSuccess) { r = fail(ES_ERR_DEVICE, "table kernel launch failed"); break; }
+      size_t tmp_bytes = 0;
+      if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, k_lo, out_lo, k_hi, out_hi, (uint32_t)n,
+                                    0, 64, m->stream) != hipSuccess ||
+          hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)) != hipSuccess) {
+        r = fail(ES_ERR_DEVICE, "radix sort set-up failed");
+        break;
+      }
+      // low word first, then the high word (LSD radix sort is stable): 128-bit order
+      if (rocprim::radix_sort_pairs(tmp, tmp_bytes, k_lo, out_lo, k_hi, out_hi, (uint32_t)n, 0,
+                                    64, m->stream) != hipSuccess ||
+          rocprim::radix_sort_pairs(tmp, tmp_bytes, out_hi, k_hi, out_lo, k_lo, (uint32_t)n, 0,
+                                    64, m->stream) != hipSuccess ||
+          hipMemcpyAsync(out_hi, k_hi, n * 8, hipMemcpyDeviceToDevice, m->stream) != hipSuccess ||
+          hipMemcpyAsync(out_lo, k_lo, n * 8, hipMemcpyDeviceToDevice, m->stream) != hipSuccess ||
+          hipStreamSynchronize(m->stream) != hipSuccess) {
+        r = fail(ES_ERR_DEVICE, "radix sort failed");
+        break;
+      }
+    } while (false);
+    hipStreamSynchronize(m->stream);
+    hipFree(k_hi);
+    hipFree(k_lo);
+    hipFree(tmp);
+    return r;
+  };
+  if ((rc = build_table(d->n2, d_radix2, d_off2, d_t2lo, d_t2hi, m->n2, m->d_s2hi, m->d_s2lo)))
+    return rc;
+  // sweep: half 1 as a sorted table too, when it and its sort buffers fit beside the rest
+  bool sweep = d->sweep == 2;
+  if (d->sweep == 0 && m->n1 < (1ull << 32)) {
+    size_t free_b = 0, total_b = 0;
+    ES_HIP(hipMemGetInfo(&free_b, &total_b));
+    const long double need = (long double)m->n1 * 40.0L + (long double)(1ull << 30);
+    sweep = need < (long double)free_b * 0.8L;
+  }
+  if (sweep) {
+    if (m->n1 >= (1ull << 32)) return fail(ES_ERR_LIMIT, "sorted half 1 above 2^32 entries");
+    if ((rc = m->alloc(&m->d_s1hi, m->n1)) || (rc = m->alloc(&m->d_s1lo, m->n1))) return rc;
+    if ((rc = build_table(d->n1, m->d_radix1, m->d_off1, m->d_t1lo, m->d_t1hi, m->n1, m->d_s1hi,
+                          m->d_s1lo)))
+      return rc;
+  }
   *out = g.release();
   return ES_OK;
 }
@@ -478,7 +615,24 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
   const uint64_t chunk = (uint64_t)std::min<long double>(std::max<long double>(per, 65536.0L),
                                                          (long double)(1ull << 26));
   ES_HIP(hipEventRecord(m->ev0, m->stream));
-  for (uint64_t s = 0; s < m->n1; s += chunk) {
+  if (m->d_s1hi) {
+    // sorted sweep: 256 consecutive s1 per wave; launches of ~2^34 expected candidates
+    const uint64_t per_wave = 256, chunks = (m->n1 + per_wave - 1) / per_wave;
+    const long double cand_chunk = run * (long double)per_wave;
+    const uint64_t per_launch = (uint64_t)std::max<long double>(
+        1.0L, std::min<long double>((long double)(1ull << 34) / std::max<long double>(cand_chunk, 1.0L),
+                                    (long double)chunks));
+    for (uint64_t c = 0; c < chunks; c += per_launch) {
+      const uint64_t c_hi = std::min<uint64_t>(chunks, c + per_launch);
+      const uint64_t blocks = std::max<uint64_t>(
+          1, std::min<uint64_t>((c_hi - c + 3) / 4, (uint64_t)m->cus * 64));
+      hipLaunchKernelGGL(es_mitm_sweep_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream, A,
+                         (const uint64_t*)m->d_s1hi, (const uint64_t*)m->d_s1lo, m->n1, per_wave,
+                         c, c_hi);
+      ES_HIP(hipGetLastError());
+    }
+  }
+  for (uint64_t s = 0; s < m->n1 && !m->d_s1hi; s += chunk) {
     const uint64_t cnt = std::min<uint64_t>(chunk, m->n1 - s);
     const uint64_t per_block = wave ? 4 : 256;
     const uint64_t blocks =

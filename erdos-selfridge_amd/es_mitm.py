@@ -34,7 +34,7 @@ class MitmDesc(C.Structure):
         ("n2", C.c_uint32), ("radix2", nat.u32p), ("term2_lo", nat.u64p), ("term2_hi", nat.u64p),
         ("n_filters", C.c_uint32), ("filt_q", nat.u32p), ("filt_off", nat.u32p),
         ("filt_bits", nat.u32p), ("n_filt_words", C.c_uint32), ("out_cap", C.c_uint64),
-        ("walk", C.c_uint32), ("reserved", C.c_uint32),
+        ("walk", C.c_uint32), ("sweep", C.c_uint32),
     ]
 
 
@@ -109,7 +109,7 @@ class MitmSieve:
     """One whole-prime wheel for k on the current GPU (es_mitm_* in libes_sieve.so)."""
 
     def __init__(self, k: int, wheel=None, mem_gb: float = 8.0, out_cap: int = 1 << 20,
-                 walk: int = 0):
+                 walk: int = 0, sweep: int = 0):
         self.k = k
         info = _info(k)
         self.info = info
@@ -178,6 +178,7 @@ class MitmSieve:
         d.filt_bits, d.n_filt_words = nat.ptr32(keep["bits"]), int(keep["bits"].size)
         d.out_cap = out_cap
         d.walk = walk
+        d.sweep = sweep
         self._keep = keep
         self.out_cap = out_cap
         h = C.c_void_p()

@@ -42,7 +42,9 @@ typedef struct es_mitm_desc {
   uint64_t out_cap;            /* survivor buffer (n mod M as lo, hi pairs; 0 = 2^20) */
   uint32_t walk;               /* 0: chosen per scan by the expected run length n2*(b-a)/M
                                   (>= 8: wave); 1: one lane per s1; 2: one wave per s1 */
-  uint32_t reserved;
+  uint32_t sweep;              /* half 1 as a sorted table too, swept with two pointers (no
+                                  per-s1 binary search): 0 when it and its sort buffers fit in
+                                  free device memory, 1 never, 2 always */
 } es_mitm_desc;
 
 /* Builds and sorts the half-2 table on the device (its size: product of radix2, < 2^32). */

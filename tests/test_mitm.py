@@ -39,17 +39,20 @@ def test_split_and_wheel_choice(k, mem):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("walk", [1, 2])
-def test_windows_equal_range_oracle(walk):
-    rng = random.Random(31 + walk)
+@pytest.mark.parametrize("walk,sweep", [(1, 1), (2, 1), (0, 2)])
+def test_windows_equal_range_oracle(walk, sweep):
+    """walk 1 / 2: lane or wave per s1 with binary searches; sweep 2: both halves sorted, two
+    run pointers per wave moving down the table."""
+    rng = random.Random(31 + walk + 10 * sweep)
     for k, mem in ((30, 1e-4), (60, 1e-3), (100, 0.01)):
-        s = MM.MitmSieve(k, mem_gb=mem, walk=walk, out_cap=1 << 20)
+        s = MM.MitmSieve(k, mem_gb=mem, walk=walk, sweep=sweep, out_cap=1 << 20)
         for _ in range(3):
             t = rng.randrange(0, 4)
             span = min(s.M, rng.randrange(1 << 20, 1 << 24))
             a = rng.randrange(0, s.M - span + 1)
             b = a + span
-            print(f"case mitm k={k} walk={walk} wheel={s.wheel} t={t} [{a}, {b})", flush=True)
+            print(f"case mitm k={k} walk={walk} sweep={sweep} wheel={s.wheel} t={t} [{a}, {b})",
+                  flush=True)
             got = s.scan(t, a, b)
             n, want = O.c_good_range(k, t * s.M + a, t * s.M + b, threads=16)
             assert got == want and len(got) == n

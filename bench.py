@@ -538,7 +538,11 @@ def bench_mitm(a) -> int:
     cand = r["candidates"]
     n_range = r["bound"]
     n1, n2 = s.sizes
-    bytes_alg = cand * 16 + n1 * 2 * max(1, math.ceil(math.log2(n2))) * 16
+    swept = s.swept
+    # table bytes: 16 per candidate, plus per s1 either one sequential 16-byte read (sorted
+    # sweep) or two binary searches of ceil(log2 S2) 16-byte probes
+    per_s1 = 16 if swept else 2 * max(1, math.ceil(math.log2(n2))) * 16
+    bytes_alg = cand * 16 + n1 * per_s1
     achieved = bytes_alg / (dev_ms / 1e3)
     cpu = None
     if not a.no_cpu:
@@ -560,11 +564,12 @@ def bench_mitm(a) -> int:
         "data": "exact search; g(k) checked against the b-file",
         "config": {"workload": f"k={k}, table budget {a.mitm_mem} GB", "wheel": s.wheel,
                    "M_bits": s.M.bit_length(), "S1": n1, "S2": n2, "bound": str(n_range),
-                   "g": str(r["g"]), "build_s": build_s},
+                   "g": str(r["g"]), "build_s": build_s, "sorted_sweep": swept},
         "candidates": cand, "candidates_per_s": cand / (dev_ms / 1e3), "device_ms": dev_ms,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes": "16 B per candidate + 2 binary searches x ceil(log2 S2) x 16 B per s1"},
+                     "bytes": "16 B per candidate + per s1: 16 B (sorted sweep) or 2 binary "
+                              "searches x ceil(log2 S2) x 16 B"},
         "cpu_baseline": cpu}), flush=True)
     return 0
 

@@ -558,10 +558,11 @@ int es_mitm_destroy(es_mitm* m) {
   return ES_OK;
 }
 
-int es_mitm_sizes(es_mitm* m, uint64_t* n1, uint64_t* n2) {
+int es_mitm_sizes(es_mitm* m, uint64_t* n1, uint64_t* n2, uint32_t* swept) {
   if (!m) return fail(ES_ERR_INVALID, "null argument");
   if (n1) *n1 = m->n1;
   if (n2) *n2 = m->n2;
+  if (swept) *swept = m->d_s1hi ? 1u : 0u;
   return ES_OK;
 }
 

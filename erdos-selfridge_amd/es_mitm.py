@@ -194,6 +194,13 @@ class MitmSieve:
                 pass
             self._h = None
 
+    @property
+    def swept(self) -> bool:
+        """True when half 1 is a sorted table swept with two pointers (no binary searches)."""
+        n1, n2, sw = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
+        nat.check(nat.lib().es_mitm_sizes(self._h, C.byref(n1), C.byref(n2), C.byref(sw)))
+        return bool(sw.value)
+
     def scan(self, t: int, a: int, b: int) -> list[int]:
         """Survivors n of [t*M + a, t*M + b), 0 <= a < b <= M, ascending."""
         M64 = (1 << 64) - 1

@@ -95,7 +95,7 @@ def lib() -> C.CDLL:
         "es_fpass_set_shape": (C.c_int, [eng, C.c_uint32, C.c_uint32]),
         "es_mitm_create": (C.c_int, [C.c_void_p, C.POINTER(eng)]),
         "es_mitm_destroy": (C.c_int, [eng]),
-        "es_mitm_sizes": (C.c_int, [eng, u64p, u64p]),
+        "es_mitm_sizes": (C.c_int, [eng, u64p, u64p, u32p]),
         "es_mitm_scan": (C.c_int, [eng, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                    C.c_uint64, u64p]),
         "es_mitm_read": (C.c_int, [eng, u64p, C.c_uint64]),

@@ -50,8 +50,8 @@ typedef struct es_mitm_desc {
 /* Builds and sorts the half-2 table on the device (its size: product of radix2, < 2^32). */
 int es_mitm_create(const es_mitm_desc* d, es_mitm** out);
 int es_mitm_destroy(es_mitm* m);
-/* Sizes: entries of half 1 (decoded) and half 2 (table). */
-int es_mitm_sizes(es_mitm* m, uint64_t* n1, uint64_t* n2);
+/* Sizes: entries of half 1 (decoded, or a sorted table when *swept = 1) and half 2 (table). */
+int es_mitm_sizes(es_mitm* m, uint64_t* n1, uint64_t* n2, uint32_t* swept);
 /* Survivors of n in [t*M + a, t*M + b), 0 <= a < b <= M (a, b as lo/hi pairs); synchronous.
  * *count may exceed the buffer (then ES_ERR_OVERFLOW). */
 int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t b_lo,

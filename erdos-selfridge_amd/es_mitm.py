@@ -246,3 +246,40 @@ def mitm_search(k: int, mem_gb: float = 8.0, bound_mult: float = 3.0, max_rounds
                     "device_ms": total_ms, "candidates": total_cand}
         B = min(B * 4, s.M)
     return {"k": k, "status": "ROUND_LIMIT", "device_ms": total_ms, "candidates": total_cand}
+
+
+def main(argv=None) -> int:
+    """CLI with the argv and JSON lines of sieve/gpu_mitm.py:225-239: one JSON object per k
+    (status, g, bound, wheel, sizes, and the b-file comparison when the k is in it)."""
+    import argparse
+    import json
+    import os
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("k", type=int, nargs="+")
+    ap.add_argument("--mem-gb", type=float, default=8.0)
+    ap.add_argument("--bound-mult", type=float, default=3.0)
+    ap.add_argument("--chunk-log2", type=int, default=24,
+                    help="accepted for argv compatibility; launches are sized by the engine")
+    ap.add_argument("--bfile", default=None, help="b-file (k n per line) to compare against")
+    a = ap.parse_args(argv)
+    bf = {}
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = a.bfile or os.path.join(os.path.dirname(here), "tests", "golden", "b003458_1to400.txt")
+    if os.path.exists(path):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 2 and t[0].isdigit():
+                bf[int(t[0])] = int(t[1])
+    for k in a.k:
+        r = mitm_search(k, mem_gb=a.mem_gb, bound_mult=a.bound_mult)
+        if r.get("status") == "FOUND":
+            r["bfile"] = str(bf[k]) if k in bf else None
+            r["matches_bfile"] = (bf[k] == r["g"]) if k in bf else None
+            r["g"] = str(r["g"])
+            r["bound"] = str(r["bound"])
+        print(json.dumps(r, default=str), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -69,3 +69,15 @@ def test_search_finds_bfile_g(k, mem):
     print(f"case mitm search k={k}: {r['status']} bound={r.get('bound')} "
           f"candidates={r['candidates']} ms={r['device_ms']:.1f}", flush=True)
     assert r["status"] == "FOUND" and r["g"] == bfile()[k]
+
+
+@pytest.mark.gpu
+def test_cli_json_lines(capsys):
+    """argv and JSON lines of the reference's MITM CLI (sieve/gpu_mitm.py:225-239)."""
+    import json
+    assert MM.main(["100", "150", "--mem-gb", "0.1"]) == 0
+    lines = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")]
+    assert [r["k"] for r in lines] == [100, 150]
+    for r in lines:
+        assert r["status"] == "FOUND" and r["matches_bfile"] is True
+        assert int(r["g"]) == bfile()[r["k"]]

@@ -22,6 +22,17 @@ int fail(int code, const std::string& msg);
 
 __host__ __device__ inline u128 mk128(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
 inline uint32_t mod_u128(u128 x, uint32_t q) { return (uint32_t)(x % q); }
+// a^-1 mod m (extended Euclid), or 0 when gcd(a, m) != 1 or m < 2
+inline uint32_t inv_mod(uint32_t a, uint32_t m) {
+  if (m < 2) return 0;
+  int64_t r0 = m, r1 = a % m, s0 = 0, s1 = 1;
+  while (r1) {
+    const int64_t qt = r0 / r1, r2 = r0 - qt * r1, s2 = s0 - qt * s1;
+    r0 = r1, r1 = r2, s0 = s1, s1 = s2;
+  }
+  if (r0 != 1) return 0;
+  return (uint32_t)(s0 < 0 ? s0 + m : s0);
+}
 
 }  // namespace es
 

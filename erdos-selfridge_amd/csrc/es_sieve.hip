@@ -16,17 +16,20 @@
 // wave walks in the same order, so the strip's partial sums (H, L) and their residues mod
 // each mask modulus are wave-uniform rows (scalar loads).  Rings m+1.. are the lane's strip
 // base u, decoded once per strip.  Per outer index a lane does one 128-bit compare (carry of
-// l into h), a few 32-bit ops, and for each word-mask prefilter one LDS row load: row
-// x = (b + tM) mod m_f holds two r0-bit masks (ring-0 wrap clear / set) of the positions whose
-// n passes the prefilter; the wrap row selects between them bit by bit.  Each mask table is
-// stored twice over (rows [0, 2m)) so x = BQ_u + Q_sigma needs no reduction, and each lane
-// keeps the LDS byte address of its two candidate rows, so a mask costs a select, an add,
-// one ds_read and one bit-select.
+// l into h), a few 32-bit ops, and for each word-mask prefilter one aligned 8-byte LDS load:
+// table f keeps the r0-bit masks of phase x = (b + tM) mod m_f (bit i: n passes the prefilter
+// at ring-0 slot i) at row x * (-M)^-1 mod m_f, as two words: the mask of x and that of
+// x - M (the phase of a slot whose inner term wraps past M), which is the next row's first
+// word.  Rows are stored twice over (rows [0, 2m)) so P(BQ_u) + P(Q_sigma) + wrap needs no
+// reduction, and each lane keeps the LDS byte address of its strip-base row, so a mask costs
+// one add, one ds_read and one 3-input AND; the wrap row selects between the two ANDed words
+// once.
 //
 // Exactness.  Prefilters are necessary conditions (low base-p digits of a filter prime), so
 // they only discard positions the full predicate rejects.  Every surviving position is then
 // re-tested with the full admissibility tables of every remaining filter from its exact
-// 128-bit residue s -- the reference predicate (sieve/gpu_sieve_v9.py:304-322).
+// 128-bit residue s -- the reference predicate (sieve/gpu_sieve_v9.py:304-322).  Mask words
+// that still have candidates are queued per wave and re-tested 64 at a time.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +55,7 @@ int fail(int code, const std::string& msg) {
 using es::fail;
 using es::mk128;
 using es::mod_u128;
+using es::inv_mod;
 
 namespace {
 
@@ -128,16 +132,91 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
   }
 }
 
+// Per-wave candidate queue in LDS, after the tables (kWaveQueueBytes per wave): up to
+// kQueueCap entries {bits, (sg * words + w) | lane << 26} of mask words that still have
+// candidates.  Entries wait until 64 are queued and are then re-tested exactly, one per lane,
+// so the exact test runs on full waves; a strip's leftovers are re-tested when it ends.  The
+// entry's lane supplies its strip data (z, outer base, h_u) through cross-lane permutes.
+constexpr uint32_t kQueueCap = 128;
+constexpr uint32_t kWaveQueueBytes = kQueueCap * 8;
+constexpr uint32_t kQueueIndexBits = 26;  // sg * words + w < 2^26 (checked at engine creation)
+
+__device__ __forceinline__ uint2* es_wave_queue(const HotPlan& P, const uint32_t* lds) {
+  const uint32_t base = (P.lds_words * 4u + 15u) & ~15u;
+  return reinterpret_cast<uint2*>(reinterpret_cast<char*>(const_cast<uint32_t*>(lds)) + base +
+                                  (threadIdx.x >> 6) * kWaveQueueBytes);
+}
+
+__device__ __forceinline__ uint32_t es_from_lane(uint32_t src, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+
+// Exact re-test of queued entries [first, first + n), one per lane.  Called by the whole wave
+// (the permutes read every lane's strip data).
+template <int W>
+__device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint32_t first,
+                                         uint32_t n, uint32_t hu, uint64_t zl, uint64_t zh,
+                                         uint64_t obase, uint64_t idx_lo, uint64_t idx_hi) {
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint2 ent = lane < n ? q[first + lane] : make_uint2(0u, 0u);
+  const uint32_t src = ent.y >> kQueueIndexBits;
+  const uint64_t szl = (uint64_t)es_from_lane(src, (uint32_t)zl) |
+                       ((uint64_t)es_from_lane(src, (uint32_t)(zl >> 32)) << 32);
+  const uint64_t szh = (uint64_t)es_from_lane(src, (uint32_t)zh) |
+                       ((uint64_t)es_from_lane(src, (uint32_t)(zh >> 32)) << 32);
+  const uint64_t sob = (uint64_t)es_from_lane(src, (uint32_t)obase) |
+                       ((uint64_t)es_from_lane(src, (uint32_t)(obase >> 32)) << 32);
+  const uint32_t shu = es_from_lane(src, hu);
+  if (lane < n) {
+    const uint32_t sw = ent.y & ((1u << kQueueIndexBits) - 1u);
+    const uint32_t nw = W ? (uint32_t)W : P.W;
+    const uint32_t sg = W == 1 ? sw : sw / nw, w = sw - sg * nw;
+    const uint32_t* row = P.strip + (size_t)sg * P.stride;
+    const uint64_t Ll = (uint64_t)row[2] | ((uint64_t)row[3] << 32);
+    const uint64_t Lh = (uint64_t)row[4] | ((uint64_t)row[5] << 32);
+    const bool kap = es_ge128(Lh, Ll, szh, szl);
+    uint32_t h = shu + row[0] + (kap ? 1u : 0u);
+    if (h >= P.q0) h -= P.q0;
+    es_exact_tail(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Queues this lane's candidate word (bits != 0); drains 64 entries once that many wait.
+// `cnt` is wave-uniform.
+template <int W>
+__device__ __forceinline__ void es_enqueue(const HotPlan& P, uint2* q, uint32_t& cnt,
+                                           uint32_t bits, uint32_t sw, uint32_t hu, uint64_t zl,
+                                           uint64_t zh, uint64_t obase, uint64_t idx_lo,
+                                           uint64_t idx_hi) {
+  const uint64_t any = __ballot(bits != 0u);
+  if (any == 0) return;
+  if (bits) {
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
+    q[cnt + below] = make_uint2(bits, sw | ((threadIdx.x & 63u) << kQueueIndexBits));
+  }
+  cnt += (uint32_t)__popcll(any);
+  if (cnt >= 64u) {
+    cnt -= 64u;
+    es_drain<W>(P, q, cnt, 64u, hu, zl, zh, obase, idx_lo, idx_hi);
+  }
+}
+
 // One strip: S outer indices, uniform across the wave.  W = words per ring-0 mask (0 = runtime).
 // The last G of the F masks are read from global memory (vector loads through L1) instead of
-// LDS, so the two load pipes share the row traffic.
+// LDS, so the two load pipes share the row traffic.  A[f] is the byte address of the lane's
+// row in table f; an outer sum that wraps past M starts one row on.  The words for slots
+// without an inner wrap and with one are ANDed over the masks first and merged by the carry
+// word once.
 template <int F, int G, int W, bool CHECK>
 __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
-                                         const uint32_t (&A0)[F + 1],
-                                         const uint32_t (&A1)[F + 1], uint32_t hu, uint64_t zl,
+                                         const uint32_t (&A)[F + 1], uint32_t hu, uint64_t zl,
                                          uint64_t zh, uint64_t obase, uint32_t sb, uint32_t se,
-                                         uint64_t idx_lo, uint64_t idx_hi) {
+                                         uint2* wq, uint64_t idx_lo, uint64_t idx_hi) {
   const uint32_t q0 = P.q0, S = P.S, stride = P.stride;
+  uint32_t cnt = 0;
   const uint32_t nw = W ? (uint32_t)W : P.W;
   const char* lb = reinterpret_cast<const char*>(lds);
   const char* gb = reinterpret_cast<const char*>(P.gmask);
@@ -152,38 +231,40 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
     const bool c1 = h >= q0;                        // b_u + B_sigma >= M
     h = c1 ? h - q0 : h;
     const bool live = !CHECK || (sg >= sb && sg < se);
+    const uint32_t wrap = c1 ? 8u * nw : 0u;        // one row on
     if (W == 1) {
       const uint32_t carry = carry_tab[h];
-      uint32_t bits = 0xffffffffu;
-      if (F == 0) bits = P.r0 >= 32 ? 0xffffffffu : ((1u << P.r0) - 1u);
+      uint32_t bx = 0xffffffffu, by = 0xffffffffu;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        const uint32_t a = (c1 ? A1[f] : A0[f]) + row[8 + f];
+        const uint32_t a = A[f] + row[8 + f] + wrap;
         const uint2 v = f < F - G ? *reinterpret_cast<const uint2*>(lb + a)
                                   : *reinterpret_cast<const uint2*>(gb + a);
-        bits &= (carry & v.y) | (~carry & v.x);
+        bx &= v.x;
+        by &= v.y;
       }
+      uint32_t bits = (carry & by) | (~carry & bx);
+      if (F == 0) bits = P.r0 >= 32 ? 0xffffffffu : ((1u << P.r0) - 1u);
       if (!live) bits = 0;
-      if (bits)
-        es_exact_tail(P.cold, bits, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
+      es_enqueue<W>(P, wq, cnt, bits, sg, hu, zl, zh, obase, idx_lo, idx_hi);
     } else if (W == 2) {
       const uint2 carry = *reinterpret_cast<const uint2*>(carry_tab + 2 * h);
-      uint32_t b0 = 0xffffffffu, b1 = 0xffffffffu;
-      if (F == 0) b1 = P.r0 >= 64 ? 0xffffffffu : ((1u << (P.r0 - 32)) - 1u);
+      uint32_t b0x = 0xffffffffu, b0y = 0xffffffffu, b1x = 0xffffffffu, b1y = 0xffffffffu;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        const uint32_t a = (c1 ? A1[f] : A0[f]) + row[8 + f];
+        const uint32_t a = A[f] + row[8 + f] + wrap;
         const uint4 v = *reinterpret_cast<const uint4*>(lb + a);
-        b0 &= (carry.x & v.y) | (~carry.x & v.x);
-        b1 &= (carry.y & v.w) | (~carry.y & v.z);
+        b0x &= v.x;
+        b0y &= v.y;
+        b1x &= v.z;
+        b1y &= v.w;
       }
+      uint32_t b0 = (carry.x & b0y) | (~carry.x & b0x);
+      uint32_t b1 = (carry.y & b1y) | (~carry.y & b1x);
+      if (F == 0) b1 = P.r0 >= 64 ? 0xffffffffu : ((1u << (P.r0 - 32)) - 1u);
       if (!live) b0 = b1 = 0;
-      if (b0 | b1) {
-        if (b0)
-          es_exact_tail(P.cold, b0, 0, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
-        if (b1)
-          es_exact_tail(P.cold, b1, 1, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
-      }
+      es_enqueue<W>(P, wq, cnt, b0, 2 * sg, hu, zl, zh, obase, idx_lo, idx_hi);
+      es_enqueue<W>(P, wq, cnt, b1, 2 * sg + 1, hu, zl, zh, obase, idx_lo, idx_hi);
     } else {
       // carrying slots are the suffix [j, r0) of the sorted ring-0 slots
       const uint32_t j = reinterpret_cast<const uint16_t*>(carry_tab)[h];
@@ -191,23 +272,25 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
       for (uint32_t w = 0; w < nw; ++w) {
         const int d = (int)j - 32 * (int)w;
         const uint32_t carry = d <= 0 ? 0xffffffffu : (d >= 32 ? 0u : 0xffffffffu << d);
-        uint32_t bits = 0xffffffffu;
+        uint32_t bx = 0xffffffffu, by = 0xffffffffu;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          const uint32_t a = A[f] + row[8 + f] + wrap + 8u * w;
+          const uint2 v = *reinterpret_cast<const uint2*>(lb + a);
+          bx &= v.x;
+          by &= v.y;
+        }
+        uint32_t bits = (carry & by) | (~carry & bx);
         if (F == 0) {
           const uint32_t n = P.r0 - 32u * w;
           bits = n >= 32u ? 0xffffffffu : ((1u << n) - 1u);
         }
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          const uint32_t a = (c1 ? A1[f] : A0[f]) + row[8 + f] + 8u * w;
-          const uint2 v = *reinterpret_cast<const uint2*>(lb + a);
-          bits &= (carry & v.y) | (~carry & v.x);
-        }
         if (!live) bits = 0;
-        if (bits)
-          es_exact_tail(P.cold, bits, w, h, kap, zl, zh, sg, obase + sg, idx_lo, idx_hi);
+        es_enqueue<W>(P, wq, cnt, bits, sg * nw + w, hu, zl, zh, obase, idx_lo, idx_hi);
       }
     }
   }
+  if (cnt) es_drain<W>(P, wq, 0, cnt, hu, zl, zh, obase, idx_lo, idx_hi);
 }
 
 // Strip u of a plan (u may be a replay of a valid strip for lanes past the range: `in` = false
@@ -251,8 +334,8 @@ __device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, 
   }
   const uint64_t zl = M0l - ll;  // z = M0 - l_u: l_u + L >= M0 <=> L >= z
   const uint64_t zh = M0h - lh - (M0l < ll ? 1ull : 0ull);
-  // LDS byte addresses of this lane's mask rows for wrap clear (A0) / set (A1)
-  uint32_t A0[F + 1], A1[F + 1];
+  // byte address of the row of this lane's strip-base phase x in each mask table
+  uint32_t A[F + 1];
 #pragma unroll
   for (int f = 0; f < F; ++f) {
     cu32_t* rec = mk + kMaskInfo * f;
@@ -260,13 +343,11 @@ __device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, 
     const uint64_t bm = (uint64_t)hu * rec[2] + es_mod128(ll, lh, m, rec[3], rec[4], rec[5]) +
                         rec[7];
     const uint32_t x = (uint32_t)(bm % m);
-    uint32_t x1 = x + rec[6];
-    if (x1 >= m) x1 -= m;
-    A0[f] = rec[1] + row_bytes * x;
-    A1[f] = rec[1] + row_bytes * x1;
+    A[f] = rec[1] + row_bytes * (uint32_t)((uint64_t)x * rec[6] % m);
   }
-  A0[F] = A1[F] = 0;
+  A[F] = 0;
   const uint64_t obase = u * S;
+  uint2* wq = es_wave_queue(P, lds);
   uint32_t sb = 0, se = 0;
   if (in) {
     sb = o_lo > obase ? (uint32_t)(o_lo - obase) : 0u;
@@ -274,13 +355,13 @@ __device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, 
   }
   const bool full = in && sb == 0 && se == S;
   if (__all(full))
-    es_sweep<F, G, W, false>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi);
+    es_sweep<F, G, W, false>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
   else
-    es_sweep<F, G, W, true>(P, lds, A0, A1, hu, zl, zh, obase, sb, se, idx_lo, idx_hi);
+    es_sweep<F, G, W, true>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
 }
 
 template <int F, int G, int W>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, uint64_t o_hi,
                 uint64_t idx_lo, uint64_t idx_hi) {
   extern __shared__ uint32_t lds[];
@@ -307,7 +388,7 @@ struct GridJob {
 };
 
 template <int F, int G, int W>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
                      uint32_t n_jobs) {
   extern __shared__ uint32_t lds[];
@@ -573,7 +654,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // ring that crosses the target is taken when S * nxt < target^2)
   while (1 + m < w) {
     const uint64_t nxt = S * d->ring_radix[1 + m];
-    if (nxt > (1u << 22)) break;
+    if (nxt > (1u << 22) || nxt * e->W >= (1u << kQueueIndexBits)) break;
     if (m > 0 && nxt > target) {
       if (nxt * S < (uint64_t)target * target) {
         S = nxt;
@@ -589,8 +670,14 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->strips = e->R_outer / e->S;
   const uint32_t nb = w - 1 - m;
 
-  // ---- mask tables: rows [0, 2m) of 2W words (c0, c1 per ring-0 word).  Staged masks go to
-  // LDS, except that for W = 1 the G smallest tables are read from global memory through L1.
+  // ---- mask tables: rows [0, 2m) of 2W words (per ring-0 word: slots without / with an inner
+  // wrap past M), one bit per sorted ring-0 slot.  Staged masks go to LDS, except that for
+  // W = 1 the G smallest tables are read from global memory through L1.  A table of modulus m
+  // keeps phase x at row P(x) = x * cinv mod m (and again at P + m), where cinv = c^-1 mod m
+  // and c = -M mod m: subtracting M from n moves every table exactly one row on, so an outer
+  // sum that wraps past M starts one row further on in every table (one shared offset), and
+  // a row's second word is the next row's first.  A mask whose m is not prime to M has no
+  // such c^-1 and is not staged (it is only a prefilter; its filter stays exact).
   const uint32_t budget_words = (d->lds_budget ? d->lds_budget : 65536u) / 4;
   // Carry layout.  W <= 2: one row of W words per h (bit i set iff h + e_i >= q0).  W > 2:
   // slots are sorted by e_i, so the carrying slots of row h are a suffix [j(h), r0); the table
@@ -604,6 +691,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     for (uint32_t f = 0; f < d->n_masks && staged.size() < (size_t)kMaxMasks; ++f) {
       const size_t rows_words = (size_t)4 * d->mask_mod[f] * W + 4;
       if (used + rows_words > budget_words) continue;  // stays exact-only
+      if (inv_mod((uint32_t)((d->mask_mod[f] - mod_u128(M, d->mask_mod[f])) % d->mask_mod[f]),
+                  d->mask_mod[f]) == 0)
+        continue;                                      // m not prime to M
       used += rows_words;
       staged.push_back(f);
     }
@@ -623,7 +713,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     if (std::find(glob.begin(), glob.end(), f) == glob.end()) order_masks.push_back(f);
   for (uint32_t f : staged)
     if (std::find(glob.begin(), glob.end(), f) != glob.end()) order_masks.push_back(f);
-  std::vector<uint32_t> lds, gimg, mbase, mmod, msrc;
+  std::vector<uint32_t> lds, gimg, mbase, mmod, msrc, minv;
   for (size_t pos = 0; pos < order_masks.size(); ++pos) {
     const uint32_t f = order_masks[pos];
     const bool in_global = pos + G >= order_masks.size();
@@ -636,13 +726,14 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     const uint32_t corr = (mq - mod_u128(M, mq)) % mq;
     std::vector<uint32_t> t0m(r0);
     for (uint32_t i = 0; i < r0; ++i) t0m[i] = mod_u128(e->terms[order[i]], mq);
-    for (uint32_t x = 0; x < mq; ++x) {
+    for (uint32_t P = 0; P < mq; ++P) {
+      const uint32_t x = (uint32_t)((uint64_t)P * corr % mq);  // phase held at row P
       for (uint32_t i = 0; i < r0; ++i) {
         const uint32_t a = (uint32_t)(((uint64_t)x + t0m[i]) % mq);
         const uint32_t b = (uint32_t)(((uint64_t)a + corr) % mq);
         const uint32_t wi = i >> 5, bit = 1u << (i & 31);
         for (uint32_t copy = 0; copy < 2; ++copy) {
-          uint32_t* row = &img[off + (size_t)2 * W * (x + copy * mq) + 2 * wi];
+          uint32_t* row = &img[off + (size_t)2 * W * (P + copy * mq) + 2 * wi];
           if ((bm[a >> 5] >> (a & 31)) & 1u) row[0] |= bit;
           if ((bm[b >> 5] >> (b & 31)) & 1u) row[1] |= bit;
         }
@@ -651,6 +742,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     mbase.push_back((uint32_t)(off * 4));
     mmod.push_back(mq);
     msrc.push_back(src);
+    minv.push_back(inv_mod(corr, mq));
   }
   if (gimg.empty()) gimg.push_back(0);
   e->F = (uint32_t)mmod.size();
@@ -671,7 +763,11 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
         if (h + e_sorted[i] >= q0) lds[carry_off + (size_t)h * W + (i >> 5)] |= 1u << (i & 31);
   }
   if (lds.size() > 40960) return fail(ES_ERR_LIMIT, "ring-0 tables exceed 160 KiB of LDS");
-  e->lds_bytes = (uint32_t)(lds.size() * 4);
+  // per workgroup: the tables, then each wave's candidate queue
+  e->lds_bytes = (uint32_t)((lds.size() * 4 + 15) & ~size_t(15)) +
+                 (e->threads / 64) * kWaveQueueBytes;
+  if ((uint64_t)e->S * W >= (1u << kQueueIndexBits))
+    return fail(ES_ERR_LIMIT, "strip rows times ring-0 words exceed the queue index");
 
   // Every filter is decided: re-tested exactly, or covered in full by a staged mask (same
   // modulus and bitmap).  Filters left out of slow_order whose mask was not staged are
@@ -732,7 +828,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     row[3] = (uint32_t)(L >> 32);
     row[4] = (uint32_t)(L >> 64);
     row[5] = (uint32_t)(L >> 96);
-    for (uint32_t f = 0; f < F; ++f) row[8 + f] = 8u * W * mod_u128(B, mmod[f]);
+    for (uint32_t f = 0; f < F; ++f)  // row offset of phase B in table f
+      row[8 + f] = 8u * W * (uint32_t)((uint64_t)mod_u128(B, mmod[f]) * minv[f] % mmod[f]);
     for (uint32_t j = 0; j < m; ++j) {  // odometer
       if (++dig[j] < d->ring_radix[1 + j]) break;
       dig[j] = 0;
@@ -750,7 +847,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     r[3] = mod_u128((u128)1 << 32, mq);
     r[4] = mod_u128((u128)1 << 64, mq);
     r[5] = mod_u128((u128)1 << 96, mq);
-    r[6] = (mq - mod_u128(M, mq)) % mq;
+    r[6] = minv[f];  // row of phase x: x * r[6] mod m
     r[7] = 0;
   }
   // ---- exact-filter records
@@ -789,7 +886,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
   if (e->lds_bytes > prop.sharedMemPerBlock)
-    return fail(ES_ERR_LIMIT, "LDS tables exceed the device's per-workgroup LDS");
+    return fail(ES_ERR_LIMIT, "LDS tables and candidate queues exceed the device's "
+                              "per-workgroup LDS (fewer threads or a smaller budget)");
   e->max_grid = 1u << 30;  // one strip per thread; the dispatcher balances the tail
   ES_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   ES_HIP(hipMalloc(&e->dblob, bl.bytes.size()));

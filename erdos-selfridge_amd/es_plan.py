@@ -45,7 +45,7 @@ TUNABLES = {
     "word_masks": "auto",      # "auto" | "force" | "off"  (accepted; masks are always exact)
     "mi_launch_log2": 37,      # positions per engine launch (outer units = 2^this / r0)
     "mi_strip_target": 1024,   # target outer indices per uniform strip
-    "mi_masks": 8,             # word-mask prefilters staged in LDS
+    "mi_masks": "auto",        # word-mask prefilters staged in LDS ("auto": by candidate rate)
     "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
     "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
     "mi_inner_radix": "auto",  # ring 0 mod p^T wider than this is swept as digit rings (0:

@@ -63,11 +63,11 @@ class PlanSpace:
         # else the radix (b = 0: never split)
         self.inner = self._inner_table(128)
         self.inner_alt = {b: self._inner_table(b) for b in SPLIT_BOUNDS}
-        # word-mask pass rate of p as a filter at wheel depth T: the deepest digit sub-filter
-        # mod p^D <= 4096, D > T (1 when there is none)
+        # word-mask pass rate of p as a filter outside the wheel: the deepest digit sub-filter
+        # mod p^D <= 4096 (1 when there is none; a wheel prime divides M and gets no mask)
         self.mrate = np.ones_like(self.cnt)
         for i, p in enumerate(self.ps):
-            for T in range(0, int(self.tp[i])):
+            for T in range(0, min(1, int(self.tp[i]))):
                 D = T
                 while D + 1 <= int(self.tp[i]) and p ** (D + 1) <= 4096:
                     D += 1

@@ -32,6 +32,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 MASK_MOD_CAP = 4096          # largest prefilter modulus staged in LDS
 MAX_MASKS = 12
+AUTO_MASK_STOP = 0.01        # "auto" masks: stop below this many candidate slots per outer index
 
 
 # --------------------------------------------------------------------------- table building
@@ -93,11 +94,16 @@ def _table_words(k: int, p: int, T: int) -> np.ndarray:
 
 
 def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
-                 lds_budget: int = 65536, r0: int | None = None, q0: int | None = None):
-    """Word-mask prefilters: low-digit conditions mod p^D (T_p < D <= t_p, p^D <= MASK_MOD_CAP)
-    of the filter primes -- each a necessary condition of its full test.  Candidates are taken
+                 lds_budget: int = 65536, r0: int | None = None, q0: int | None = None,
+                 stop: float = 0.0):
+    """Word-mask prefilters: low-digit conditions mod p^D (D <= t_p, p^D <= MASK_MOD_CAP) of the
+    filter primes outside the wheel -- each a necessary condition of its full test (a wheel
+    prime divides M, and the engine's mask tables need M invertible mod p^D).  Candidates are taken
     by least conditional pass rate, one depth per prime, while their LDS rows (each table held
     twice over, two words per ring-0 word per row) fit the budget beside the ring-0 carry table.
+    With `stop` > 0, masks are added only while the expected candidate slots per outer index
+    (r0 x the pass rates taken) is at least `stop`: below that, a further mask costs more LDS
+    reads than the queued exact re-test it saves.
     Returns [(filter_index, D, modulus, conditional pass rate)]."""
     k = plan.k
     rest = plan.rest if limit is None else plan.rest[:limit]
@@ -109,8 +115,10 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     cands = []
     for fi, p in enumerate(rest):
         T = plan.wheelT.get(p, 0)
-        base = partial_count(k, p, T) / p ** T if T else 1.0
-        for d in range(T + 1, plan.tp[p] + 1):
+        if T:
+            continue
+        base = 1.0
+        for d in range(1, plan.tp[p] + 1):
             if p ** d > MASK_MOD_CAP:
                 break
             rate = (partial_count(k, p, d) / p ** d) / base
@@ -120,14 +128,16 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     carry = (q0 + 1) // 2 if words > 2 else q0 * words
     left = lds_budget // 4 - carry - 8
     taken, out = set(), []
+    dens = float(r0)
     for rate, fi, d, m in cands:
-        if len(out) >= n_masks:
+        if len(out) >= n_masks or dens < stop:
             break
         cost = 4 * words * m + 4      # rows [0, 2m), two words per ring-0 word
         if fi in taken or cost > left:
             continue
         taken.add(fi)
         left -= cost
+        dens *= rate
         out.append((fi, d, m, rate))
     return out
 
@@ -182,9 +192,11 @@ class _Desc:
         self.filt_q = np.asarray(qs or [2], dtype=np.uint32)
         self.filt_off = np.asarray(offs or [0], dtype=np.uint32)
         self.filt_bits = np.concatenate(words) if words else np.zeros(1, dtype=np.uint32)
-        n_masks = min(MAX_MASKS, int(tun.get("mi_masks", 8)))
-        self.masks = choose_masks(plan, n_masks, limit, int(tun.get("mi_lds_budget", 65536)),
-                                  self.r0, self.q0)
+        nm = tun.get("mi_masks", "auto")
+        auto = nm == "auto"
+        self.masks = choose_masks(plan, MAX_MASKS if auto else min(MAX_MASKS, int(nm)), limit,
+                                  int(tun.get("mi_lds_budget", 65536)), self.r0, self.q0,
+                                  AUTO_MASK_STOP if auto else 0.0)
         self.slow = slow_order(plan, self.masks, limit)
         mm, ms, mo, mw = [], [], [], []
         nw = 0

@@ -89,7 +89,7 @@ typedef struct es_engine_info {
   uint32_t r0, q0;             /* ring-0 radix and modulus */
   uint32_t strip_len;          /* outer indices per strip */
   uint32_t n_masks_used;       /* masks staged in LDS (may be < n_masks if LDS is short) */
-  uint32_t lds_bytes;          /* LDS tables per workgroup */
+  uint32_t lds_bytes;          /* LDS per workgroup: tables and per-wave candidate queues */
   uint32_t words;              /* u32 words per ring-0 mask row */
   uint32_t threads;
   uint32_t device;

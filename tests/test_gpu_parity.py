@@ -164,8 +164,8 @@ def test_lds_short_masks_stay_exact(monkeypatch):
     import gpu_sieve_mi355x as G
     orig = G.choose_masks
     monkeypatch.setattr(G, "choose_masks",
-                        lambda plan, n, limit=None, lds_budget=0, r0=None, q0=None:
-                        orig(plan, n, limit, 1 << 30, r0, q0))
+                        lambda plan, n, limit=None, lds_budget=0, r0=None, q0=None, stop=0.0:
+                        orig(plan, n, limit, 1 << 30, r0, q0, stop))
     for k, limit in ((100, 5), (101, None)):
         plan = EP.Plan(k)
         span = min(1 << 21, plan.R)

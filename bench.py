@@ -126,16 +126,21 @@ def main() -> int:
     ops_ref = 12 + 10 * ef                              # reference-algorithm ops per position
     kern_rate = dev_pos / (dev_ms / 1e3)
     F = eng.info.n_masks_used
-    # this kernel's VALU lane-ops per position: 55 VALU instructions per lane per outer index
-    # (r0 positions), measured with rocprofv3 SQ_INSTS_VALU for es_sieve_kernel<8, 0, 1> on this
-    # wheel; other instances are not profiled
-    ops_kernel = 55.0 / eng.r0 if (F == 8 and eng.info.words == 1) else None
+    # Profiled figures of es_sieve_kernel<6, 0, 1> on this wheel (rocprofv3, 2^36-position
+    # launches): 32.3 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
+    # index = r0 positions per lane), LDS busy 92 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
+    # 58 % bank-conflict stalls), HBM traffic 1.37 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
+    # copy-calibrated).  Other instances are not profiled.
+    profiled = F == 6 and eng.info.words == 1
+    ops_kernel = 32.3 / eng.r0 if profiled else None
     achieved = ops_ref * kern_rate
     roofline = {"bound": "valu-int32", "achieved": achieved / 1e12, "peak": INT32_PEAK / 1e12,
-                "unit": "TOP/s", "frac": achieved / INT32_PEAK, "traffic": None,
+                "unit": "TOP/s", "frac": achieved / INT32_PEAK,
+                "traffic": 1.37e6 if profiled else None,
                 "ops_per_position_reference": round(ops_ref, 3),
                 "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
                 "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
+                "lds_busy_frac_profiled": 0.92 if profiled else None,
                 "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
                 "kernel_positions_per_s": kern_rate}
 
@@ -418,7 +423,7 @@ def bench_surrogate(a) -> int:
     ks = list(range(350, 401, a.k_step))
     SHORTLISTS = (1, 2, 4, 8, 32)
     t0 = time.perf_counter()
-    X, Y, meta = S.training_set(ks, per_k=24, n_candidates=20000, seed=5,
+    X, Y, meta = S.training_set(ks, per_k=24, n_candidates=20000, seed=5, probe_log2=35,
                                 log=lambda m: print(m, file=sys.stderr, flush=True))
     t_data = time.perf_counter() - t0
     n = len(X)
@@ -438,7 +443,7 @@ def bench_surrogate(a) -> int:
     N = 1 << 36
     rows = []
     for k in ks:
-        res = S.surrogate_plan(k, model, n_candidates=a.candidates, shortlist=32, probe_log2=34,
+        res = S.surrogate_plan(k, model, n_candidates=a.candidates, shortlist=32, probe_log2=36,
                                seed=k, compare=SHORTLISTS)
         g = bf[k]
         out = {}
@@ -489,7 +494,7 @@ def bench_surrogate(a) -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "u128/u32 integer; fp32 network",
         "data": "b-file g(k); wheels enumerated per k; rates measured on 2^36-position windows",
         "config": {"workload": f"k={ks[0]}..{ks[-1]} step {a.k_step}, {a.candidates} candidates "
-                               "per k, shortlist 32 + plain, probes of 2^34 positions"},
+                               "per k, shortlist 32 + plain, probes of up to 2^36 positions"},
         "geomean_ratio": float(np.exp(np.mean(np.log(ratios)))),
         "geomean_expected_ratio": float(np.exp(np.mean(np.log(eratios)))),
         "shortlist_sizes": {

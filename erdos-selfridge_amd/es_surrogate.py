@@ -366,10 +366,11 @@ class Surrogate:
 
 
 def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
-                 log=None):
+                 log=None, probe_log2: int = 35):
     """Wheels the sieve measures itself: per k, a quarter from the 64 lowest-E_pos candidates,
     half from the 2000 lowest (the band a shortlist is drawn from) and a quarter from the 20000
-    lowest; labels (log2 density, log2 measured ns per position)."""
+    lowest; labels (log2 density, log2 measured ns per position on windows of up to
+    2^probe_log2 positions -- wheels with long strips fill the GPU only on large windows)."""
     X, Y, meta = [], [], []
     rng = random.Random(seed)
     for k in ks:
@@ -388,7 +389,7 @@ def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
         feats = sp.features(E[picks])
         dens = sp.log2_density(E[picks])
         for j, i in enumerate(picks):
-            ns = probe(k, sp.wheel(E[i]))
+            ns = probe(k, sp.wheel(E[i]), probe_log2)
             if ns is None:
                 continue
             X.append(feats[j])
@@ -400,7 +401,7 @@ def training_set(ks, per_k: int = 24, n_candidates: int = 20000, seed: int = 0,
 
 
 def surrogate_plan(k: int, model: Surrogate | None, n_candidates: int = 100_000,
-                   shortlist: int = 32, probe_log2: int = 34, seed: int = 0, log=None,
+                   shortlist: int = 32, probe_log2: int = 36, seed: int = 0, log=None,
                    compare: tuple = ()) -> dict:
     """Rank n_candidates wheels by log2 E_pos + predicted log2 ns, probe the first `shortlist`
     and the plain wheel, and return the least E_pos * measured ns (the plain wheel when no

@@ -26,7 +26,10 @@ WHEEL = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137,
          [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]]
 BLOCK = 6
 G399 = 1463367633345557449307696345739199
-INT32_PEAK = 256 * 64 * 2.4e9      # CUs x lanes x clock: one 32-bit VALU op per lane per clock
+# INT32 VALU peak: 256 CUs x 128 lane-ops per CU-clock (two wave64 VALU instructions per clock,
+# the vector rate behind the 157.3 TFLOP/s FP32 figure) x 2.4 GHz.  `--workload valu` measures
+# 7.36e13 v_add_u32/s on this device (1.88 wave-instructions per CU-cycle at ~2.35 GHz).
+INT32_PEAK = 256 * 128 * 2.4e9
 
 
 def main() -> int:
@@ -43,11 +46,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="batch, k200: autotune the launch shape of each wheel first")
-    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "batch", "k200",
-                                           "surrogate", "mitm"),
+    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "stream", "valu", "batch",
+                                           "k200", "surrogate", "mitm"),
                     default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
                          "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters); "
+                         "valu: INT32 VALU peak microbenchmark; stream: what library "
+                         "streaming kernels reach on this device's HBM; "
                          "batch: k=300..320 batched launches; k200: full-power wheel window; "
                          "surrogate: learned planner vs plain planner, k=350..400; "
                          "mitm: bounded n-window search for g(k)")
@@ -73,6 +78,10 @@ def main() -> int:
         return bench_surrogate(a)
     if a.workload == "mitm":
         return bench_mitm(a)
+    if a.workload == "valu":
+        return bench_valu(a)
+    if a.workload == "stream":
+        return bench_stream(a)
     if a.workload == "copy":
         import torch
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")       # 1 GiB
@@ -235,6 +244,57 @@ def bench_fpass(a) -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_candidate": bytes_per, "kernel_avg_launch_ms": dev_ms},
         "cpu_baseline": cpu}), flush=True)
+    return 0
+
+
+def bench_valu(a) -> int:
+    """INT32 VALU peak of this device: 8 independent v_add_u32 chains per thread, 16 workgroups of
+    256 threads per CU (es_calib_valu).  Confirms the peak the sieve roofline is priced at."""
+    import ctypes as C
+
+    import es_native as nat
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    ms, ops = C.c_double(0), C.c_uint64(0)
+    best = 0.0
+    for _ in range(max(1, a.steps)):
+        nat.check(nat.lib().es_calib_valu(cus * 16, 4096, C.byref(ms), C.byref(ops)))
+        best = max(best, ops.value / (ms.value * 1e-3))
+    print(json.dumps({
+        "metric": "int32 VALU adds/s (v_add_u32 chains)", "value": best, "unit": "ops/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": ms.value,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic", "config": {"workload": f"{cus} CUs x 16 workgroups x 256 threads, "
+                                                    "4096 rounds of 256 adds"},
+        "peak": INT32_PEAK, "frac": best / INT32_PEAK,
+        "peak_one_instruction_per_cu_clock": INT32_PEAK / 2,
+        "cus": cus}), flush=True)
+    return 0
+
+
+def bench_stream(a) -> int:
+    """Streaming rates of library kernels on 16 GiB (torch's sum: read only; device copy:
+    read + write), HIP events, best of `steps` -- the practical HBM ceiling the filter pass is
+    compared with."""
+    import torch
+    n = 16 * (1 << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for name, fn, nbytes in (("sum_read", lambda: torch.sum(x), 4 * n),
+                             ("copy_read_write", lambda: y.copy_(x), 8 * n)):
+        fn()
+        best = float("inf")
+        for _ in range(max(1, a.steps)):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name] = {"ms": best, "GB_per_s": nbytes / (best * 1e-3) / 1e9}
+    print(json.dumps({"workload": "stream", "bytes": 4 * n, "peak_GB_per_s": HBM_PEAK / 1e9,
+                      **out}), flush=True)
     return 0
 
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libes_sieve.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("es_sieve.hip", "es_filter.hip", "es_mlp.hip",
-                                                    "es_mitm.hip")]
+                                                    "es_mitm.hip", "es_calib.hip")]
 ARCH = os.environ.get("ES_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

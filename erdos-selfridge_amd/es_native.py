@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
                                    C.c_uint64, u64p]),
         "es_mitm_read": (C.c_int, [eng, u64p, C.c_uint64]),
         "es_mitm_last": (C.c_int, [eng, C.POINTER(C.c_double), u64p]),
+        "es_calib_valu": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), u64p]),
         "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
         "es_mlp_destroy": (C.c_int, [eng]),
         "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),

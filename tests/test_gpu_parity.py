@@ -206,3 +206,38 @@ def test_overflow_is_reported():
     eng, plan = engine(30, C1_WHEEL, max_survivors=1000)
     with pytest.raises(RuntimeError, match="overflow"):
         eng.scan_window(0, 0, 1 << 20)
+
+
+@pytest.mark.parametrize("k", [100, 399])
+def test_mask_counts_agree(k):
+    """Every mask count scans the same survivors: 0 masks (every ring-0 slot queued for the
+    exact re-test), 1, 3, the automatic count and 12 (dense prefix windows, against the
+    oracle)."""
+    wheel = K399_PRIMARY if k == 399 else None
+    plan = EP.Plan(k, None, wheel)
+    rng = random.Random(k + 1)
+    span = min(1 << 21, plan.R // 2)
+    lo = rng.randrange(0, plan.R - span)
+    for masks in (0, 1, 3, "auto", 12):
+        window_check(k, wheel, 2, lo, lo + span, limit=6, mi_masks=masks)
+
+
+def test_masks_not_prime_to_M_are_skipped(monkeypatch):
+    """A mask of a wheel prime (its modulus shares a factor with M) has no row order in which
+    a wrap past M is one row on; the library must leave it unstaged and keep its filter in the
+    exact re-test, with survivors unchanged."""
+    import gpu_sieve_mi355x as G
+    plan = EP.Plan(30)
+    wheel_p = [p for p in plan.rest if plan.wheelT.get(p, 0) and plan.wheelT[p] < plan.tp[p]]
+    assert wheel_p, "k=30 default plan has a partial wheel prime"
+    p = wheel_p[0]
+    fi = plan.rest.index(p)
+    extra = (fi, plan.wheelT[p] + 1, p ** (plan.wheelT[p] + 1), 0.5)
+    orig = G.choose_masks
+    monkeypatch.setattr(G, "choose_masks", lambda *a, **kw: [extra] + orig(*a, **kw))
+    eng, _ = engine(30, None)
+    offered = len(eng._d.masks)
+    say(f"case k=30 extra mask {extra[2]} (p={p} in the wheel): offered {offered}, "
+        f"staged {eng.info.n_masks_used}")
+    assert eng._d.masks[0] == extra and eng.info.n_masks_used == offered - 1
+    window_check(30, None, 1, 0, min(1 << 22, plan.R))

@@ -14,6 +14,7 @@ typedef unsigned __int128 u128;
 // scalar (SMEM) loads when the address is wave-uniform, and assumes the data is read-only for
 // the kernel's lifetime.
 typedef __attribute__((address_space(4))) const uint32_t cu32_t;
+typedef __attribute__((address_space(4))) const double cf64_t;
 
 namespace es {
 

@@ -83,7 +83,7 @@ __device__ __forceinline__ uint64_t mitm_lower_bound(const MitmArgs& A, u128 x) 
 __device__ __forceinline__ bool mitm_test(const MitmArgs& A, u128 s) {
   const uint64_t lo = (uint64_t)s, hi = (uint64_t)(s >> 64);
   for (uint32_t j = 0; j < A.nf; ++j) {
-    const uint32_t* f = A.frec + kFiltRec * j;
+    cu32_t* f = (cu32_t*)A.frec + kFiltRec * j;  // uniform record: scalar loads
     const uint32_t q = f[0];
     uint32_t r = es_mod128(lo, hi, q, f[2], f[3], f[4]) + f[5];
     if (r >= q) r -= q;
@@ -104,14 +104,14 @@ __device__ __forceinline__ bool mitm_test_fp(const MitmArgs& A, u128 s, bool ali
   for (uint32_t j = j0; j < j1; ++j) {
     if (EXIT == 1 && !__any(alive)) break;
     if (EXIT == 2 && !alive) break;
-    const double* f = A.fdrec + 8 * j;
+    cf64_t* f = (cf64_t*)A.fdrec + 8 * j;  // uniform record: scalar loads
     const double acc = fma(d3, f[2], fma(d2, f[1], fma(d1, f[0], d0 + f[5])));
     const double q = f[3];
     double r = fma(-floor(acc * f[4]), q, acc);
     r = r < 0.0 ? r + q : r;
     r = r >= q ? r - q : r;
     const uint32_t ri = (uint32_t)r;
-    const uint32_t off = A.frec[kFiltRec * j + 1];
+    const uint32_t off = ((cu32_t*)A.frec)[kFiltRec * j + 1];
     alive = alive && ((A.fbits[off + (ri >> 5)] >> (ri & 31u)) & 1u);
   }
   return alive;
@@ -224,18 +224,23 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
   if (tested) atomicAdd(A.tested, tested);
 }
 
-// Sorted sweep: half 1 is a sorted table too.  A wave takes `chunk` consecutive s1 of it; as
-// s1 grows, (a - s1) mod M and (a + len - s1) mod M fall, so the two run ends move down the
-// half-2 table by short linear steps; a binary search is needed only at the chunk's first s1
-// and where an end wraps past 0 or M.  Runs are walked by the wave as in es_mitm_kernel<true>.
+// Sorted sweep: half 1 is a sorted table too.  A wave takes `chunk` consecutive s1 of it,
+// split into 64/GL contiguous slices, one per group of GL lanes.  The groups step through their
+// slices in lockstep, one s1 each per step, so the pointer loads of all groups share one
+// latency; each group then walks its s1's run with lane stride GL (the walk loop runs to the
+// longest run of the step, wave-uniform).  As s1 grows, (a - s1) mod M and (a + len - s1) mod M
+// fall, so a group's two run ends move down the half-2 table by short linear steps; a binary
+// search is needed only at a slice's first s1 and where an end wraps past 0 or M.
+template <int GL>
 __global__ void __launch_bounds__(256)
 es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
                      const uint64_t* __restrict__ s1lo, uint64_t n1, uint64_t chunk,
                      uint64_t c_lo, uint64_t c_hi) {
+  constexpr uint32_t NG = 64 / GL;
   const u128 M = es::mk128(A.M_lo, A.M_hi);
   const u128 a = es::mk128(A.a_lo, A.a_hi), len = es::mk128(A.len_lo, A.len_hi);
   __shared__ uint4 q_all[4 * 64];
-  const uint32_t lane = __lane_id();
+  const uint32_t lane = __lane_id(), gl = lane % GL, grp = lane / GL;
   uint4* qv = q_all + (threadIdx.x >> 6) * 64u;
   const bool fp = A.fdrec != nullptr;
   unsigned long long tested = 0;
@@ -253,68 +258,72 @@ es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  auto walk = [&](u128 s1, uint64_t st, uint64_t en) {
-    for (uint64_t u0 = st; u0 < en; u0 += 64) {  // wave-uniform bound
-      const uint64_t u = u0 + lane;
-      const bool ok = u < en;
-      u128 s = 0;
-      if (ok) {
-        s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
-        if (s >= M) s -= M;
-        ++tested;
-      }
-      if (!fp) {
-        if (ok && mitm_test(A, s)) mitm_emit(A, s);
-        continue;
-      }
-      const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
-      const unsigned long long mask = __ballot(alive);
-      const uint32_t n = __popcll(mask);
-      if (cnt + n > 64u) flush();
-      if (alive) {
-        const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
-        qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
-                              (uint32_t)(s >> 96));
-      }
-      cnt += n;
-    }
-  };
   auto ge = [&](uint64_t u, u128 x) {  // table[u] >= x
     const uint64_t h = A.s2hi[u], l = A.s2lo[u];
     const uint64_t xh = (uint64_t)(x >> 64), xl = (uint64_t)x;
     return h > xh || (h == xh && l >= xl);
   };
+  const uint64_t slice = (chunk + NG - 1) / NG;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   for (uint64_t c = c_lo + (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       c < c_hi && c * chunk < n1; c += waves) {
-    const uint64_t i_lo = c * chunk, i_hi = n1 - i_lo < chunk ? n1 : i_lo + chunk;
+       c < c_hi && c * chunk < n1; c += waves) {  // wave-uniform
+    const uint64_t w_lo = c * chunk, w_hi = n1 - w_lo < chunk ? n1 : w_lo + chunk;
+    const uint64_t g_lo = w_lo + grp * slice < w_hi ? w_lo + grp * slice : w_hi;
+    const uint64_t g_hi = w_hi - g_lo < slice ? w_hi : g_lo + slice;
     uint64_t pX = 0, pY = 0;
     u128 Xp = 0, Tp = 0;
     bool wp = false;
-    for (uint64_t i = i_lo; i < i_hi; ++i) {
-      const u128 s1 = es::mk128(s1lo[i], s1hi[i]);
-      const u128 X = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
-      const u128 Y = X + len;                          // < 2M
-      const bool w = Y > M;
-      const u128 T = w ? Y - M : Y;                    // end of the run's last part, <= M
-      if (i == i_lo || X > Xp) {
-        pX = mitm_lower_bound(A, X);
-      } else {
-        while (pX > 0 && ge(pX - 1, X)) --pX;
+    for (uint64_t step = 0; step < slice; ++step) {  // wave-uniform
+      const uint64_t i = g_lo + step;
+      u128 s1 = 0;
+      uint64_t st = 0, L = 0;  // run: L table entries from st, circular
+      if (i < g_hi) {
+        s1 = es::mk128(s1lo[i], s1hi[i]);
+        const u128 X = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
+        const u128 Y = X + len;                          // < 2M
+        const bool w = Y > M;
+        const u128 T = w ? Y - M : Y;                    // end of the run's last part, <= M
+        if (step == 0 || X > Xp) {
+          pX = mitm_lower_bound(A, X);
+        } else {
+          while (pX > 0 && ge(pX - 1, X)) --pX;
+        }
+        if (step == 0 || w != wp || T > Tp) {
+          pY = T == M ? A.n2 : mitm_lower_bound(A, T);
+        } else {
+          while (pY > 0 && ge(pY - 1, T)) --pY;
+        }
+        Xp = X;
+        Tp = T;
+        wp = w;
+        st = pX;
+        L = w ? (A.n2 - pX) + pY : pY - pX;
       }
-      if (i == i_lo || w != wp || T > Tp) {
-        pY = T == M ? A.n2 : mitm_lower_bound(A, T);
-      } else {
-        while (pY > 0 && ge(pY - 1, T)) --pY;
-      }
-      Xp = X;
-      Tp = T;
-      wp = w;
-      if (!w) {
-        walk(s1, pX, pY);
-      } else {
-        walk(s1, pX, A.n2);
-        walk(s1, 0, pY);
+      for (uint64_t k0 = 0; __ballot(k0 < L) != 0; k0 += GL) {  // wave-uniform
+        const uint64_t k = k0 + gl;
+        const bool ok = k < L;
+        u128 s = 0;
+        if (ok) {
+          uint64_t u = st + k;
+          if (u >= A.n2) u -= A.n2;
+          s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
+          if (s >= M) s -= M;
+          ++tested;
+        }
+        if (!fp) {
+          if (ok && mitm_test(A, s)) mitm_emit(A, s);
+          continue;
+        }
+        const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
+        const unsigned long long mask = __ballot(alive);
+        const uint32_t n = __popcll(mask);
+        if (cnt + n > 64u) flush();
+        if (alive) {
+          const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+          qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
+                                (uint32_t)(s >> 96));
+        }
+        cnt += n;
       }
     }
   }
@@ -399,7 +408,9 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
   es_mitm* m = g.get();
   m->M = M;
   m->w1 = d->n1;
-  if (d->walk > 2) return fail(ES_ERR_INVALID, "walk must be 0, 1 or 2");
+  if (d->walk > 2 && d->walk != 4 && d->walk != 8 && d->walk != 16 && d->walk != 32 &&
+      d->walk != 64)
+    return fail(ES_ERR_INVALID, "walk must be 0, 1, 2, 4, 8, 16, 32 or 64");
   m->wave = d->walk;
   // half 1: radices, offsets, terms
   std::vector<uint32_t> off1(d->n1);
@@ -611,14 +622,19 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
   // expected run length n2 * len / M decides the walk when not fixed, and bounds each launch
   // to ~2^34 candidates (long scans become many launches of bounded length)
   const long double run = (long double)m->n2 * (long double)len / (long double)m->M;
-  bool wave = m->wave == 2 || (m->wave == 0 && run >= 8.0L);
+  bool wave = m->wave == 2 || ((m->wave == 0 || m->wave >= 4) && run >= 8.0L);
   const long double per = (long double)(1ull << 34) / (run > 1.0L ? run : 1.0L);
   const uint64_t chunk = (uint64_t)std::min<long double>(std::max<long double>(per, 65536.0L),
                                                          (long double)(1ull << 26));
   ES_HIP(hipEventRecord(m->ev0, m->stream));
   if (m->d_s1hi) {
-    // sorted sweep: 256 consecutive s1 per wave; launches of ~2^34 expected candidates
-    const uint64_t per_wave = 256, chunks = (m->n1 + per_wave - 1) / per_wave;
+    // sorted sweep: 1024 consecutive s1 per wave in 64 / GL slices (GL lanes walk one s1's run:
+    // 4 while runs stay below 256 entries -- more concurrent s1 per wave beat shorter walks --
+    // or `walk` when it names a size); launches of ~2^34 expected candidates
+    uint32_t GL = m->wave >= 4 ? m->wave : 4;
+    if (m->wave < 4)
+      while (GL < 64 && (long double)GL * 64.0L <= run) GL *= 2;
+    const uint64_t per_wave = 1024, chunks = (m->n1 + per_wave - 1) / per_wave;
     const long double cand_chunk = run * (long double)per_wave;
     const uint64_t per_launch = (uint64_t)std::max<long double>(
         1.0L, std::min<long double>((long double)(1ull << 34) / std::max<long double>(cand_chunk, 1.0L),
@@ -627,7 +643,11 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
       const uint64_t c_hi = std::min<uint64_t>(chunks, c + per_launch);
       const uint64_t blocks = std::max<uint64_t>(
           1, std::min<uint64_t>((c_hi - c + 3) / 4, (uint64_t)m->cus * 64));
-      hipLaunchKernelGGL(es_mitm_sweep_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream, A,
+      auto kern = GL == 64 ? es_mitm_sweep_kernel<64>
+                  : GL == 32 ? es_mitm_sweep_kernel<32>
+                  : GL == 16 ? es_mitm_sweep_kernel<16>
+                  : GL == 8 ? es_mitm_sweep_kernel<8> : es_mitm_sweep_kernel<4>;
+      hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(256), 0, m->stream, A,
                          (const uint64_t*)m->d_s1hi, (const uint64_t*)m->d_s1lo, m->n1, per_wave,
                          c, c_hi);
       ES_HIP(hipGetLastError());

@@ -41,7 +41,9 @@ typedef struct es_mitm_desc {
   uint32_t n_filt_words;
   uint64_t out_cap;            /* survivor buffer (n mod M as lo, hi pairs; 0 = 2^20) */
   uint32_t walk;               /* 0: chosen per scan by the expected run length n2*(b-a)/M
-                                  (>= 8: wave); 1: one lane per s1; 2: one wave per s1 */
+                                  (>= 8: wave); 1: one lane per s1; 2: one wave per s1.  In
+                                  the sorted sweep: lanes per s1 (4, 8, 16, 32 or 64; 0, 1, 2:
+                                  chosen by the expected run length) */
   uint32_t sweep;              /* half 1 as a sorted table too, swept with two pointers (no
                                   per-s1 binary search): 0 when it and its sort buffers fit in
                                   free device memory, 1 never, 2 always */

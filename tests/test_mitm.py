@@ -39,10 +39,10 @@ def test_split_and_wheel_choice(k, mem):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("walk,sweep", [(1, 1), (2, 1), (0, 2)])
+@pytest.mark.parametrize("walk,sweep", [(1, 1), (2, 1), (0, 2), (8, 2), (16, 2), (64, 2)])
 def test_windows_equal_range_oracle(walk, sweep):
     """walk 1 / 2: lane or wave per s1 with binary searches; sweep 2: both halves sorted, two
-    run pointers per wave moving down the table."""
+    run pointers per group of `walk` lanes (0: by run length) moving down the table."""
     rng = random.Random(31 + walk + 10 * sweep)
     for k, mem in ((30, 1e-4), (60, 1e-3), (100, 0.01)):
         s = MM.MitmSieve(k, mem_gb=mem, walk=walk, sweep=sweep, out_cap=1 << 20)
@@ -60,6 +60,12 @@ def test_windows_equal_range_oracle(walk, sweep):
         a = s.M - (1 << 20) if s.M > (1 << 20) else 0
         got = s.scan(1, a, s.M)
         assert got == O.c_good_range(k, s.M + a, 2 * s.M, threads=16)[1]
+        if s.M < (1 << 24):
+            # long runs (half the table and all of it per s1), and a run crossing the table end
+            for lo, hi in ((s.M // 3, s.M // 3 + s.M // 2), (0, s.M), (s.M // 5, s.M)):
+                print(f"case mitm k={k} walk={walk} sweep={sweep} t=2 [{lo}, {hi})", flush=True)
+                got = s.scan(2, lo, hi)
+                assert got == O.c_good_range(k, 2 * s.M + lo, 2 * s.M + hi, threads=16)[1]
 
 
 @pytest.mark.gpu

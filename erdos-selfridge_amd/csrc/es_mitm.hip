@@ -224,13 +224,14 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
   if (tested) atomicAdd(A.tested, tested);
 }
 
-// Sorted sweep: half 1 is a sorted table too.  A wave takes `chunk` consecutive s1 of it,
-// split into 64/GL contiguous slices, one per group of GL lanes.  The groups step through their
-// slices in lockstep, one s1 each per step, so the pointer loads of all groups share one
-// latency; each group then walks its s1's run with lane stride GL (the walk loop runs to the
-// longest run of the step, wave-uniform).  As s1 grows, (a - s1) mod M and (a + len - s1) mod M
-// fall, so a group's two run ends move down the half-2 table by short linear steps; a binary
-// search is needed only at a slice's first s1 and where an end wraps past 0 or M.
+// Sorted sweep: half 1 is a sorted table too.  A wave takes `chunk` consecutive s1 of it and
+// its 64/GL groups of GL lanes step through them in lockstep: at each step group g takes the
+// g-th of the next 64/GL consecutive s1, so the runs of one step nearly coincide and the wave
+// reads one small, cache-resident region of the half-2 table.  Each group walks its s1's run
+// with lane stride GL (the walk loop runs to the longest run of the step, wave-uniform).  As
+// s1 grows, (a - s1) mod M and (a + len - s1) mod M fall, so a group's two run ends move down
+// the table by short steps, which its GL lanes take GL entries at a time; a binary search is
+// needed only at the chunk's first step and where an end wraps past 0 or M.
 template <int GL>
 __global__ void __launch_bounds__(256)
 es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
@@ -263,36 +264,49 @@ es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
     const uint64_t xh = (uint64_t)(x >> 64), xl = (uint64_t)x;
     return h > xh || (h == xh && l >= xl);
   };
-  const uint64_t slice = (chunk + NG - 1) / NG;
+  // Lowers p (a lower bound of x in the table that x has since fallen below) to the lower
+  // bound of x: the group's lanes test GL entries below p per round (wave-uniform rounds).
+  auto lower = [&](uint64_t& p, u128 x, bool moving) {
+    const uint64_t gmask = GL == 64 ? ~0ull : ((1ull << GL) - 1ull);
+    while (__ballot(moving) != 0) {
+      const bool c = moving && p > gl && ge(p - 1 - gl, x);
+      const uint64_t bits = (__ballot(c) >> (grp * GL)) & gmask;
+      if (moving) {
+        // entries >= x just below p (a prefix of the GL tested, the table being sorted)
+        const uint32_t m = bits == gmask ? (uint32_t)GL : (uint32_t)__builtin_ctzll(~bits);
+        p -= m;
+        moving = m == GL;
+      }
+    }
+  };
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   for (uint64_t c = c_lo + (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
        c < c_hi && c * chunk < n1; c += waves) {  // wave-uniform
     const uint64_t w_lo = c * chunk, w_hi = n1 - w_lo < chunk ? n1 : w_lo + chunk;
-    const uint64_t g_lo = w_lo + grp * slice < w_hi ? w_lo + grp * slice : w_hi;
-    const uint64_t g_hi = w_hi - g_lo < slice ? w_hi : g_lo + slice;
+    const uint64_t steps = (w_hi - w_lo + NG - 1) / NG;
     uint64_t pX = 0, pY = 0;
     u128 Xp = 0, Tp = 0;
     bool wp = false;
-    for (uint64_t step = 0; step < slice; ++step) {  // wave-uniform
-      const uint64_t i = g_lo + step;
-      u128 s1 = 0;
-      uint64_t st = 0, L = 0;  // run: L table entries from st, circular
-      if (i < g_hi) {
+    for (uint64_t step = 0; step < steps; ++step) {  // wave-uniform
+      const uint64_t i = w_lo + step * NG + grp;
+      const bool have = i < w_hi;
+      u128 s1 = 0, X = 0, T = 0;
+      bool w = false, searchX = false, searchY = false;
+      if (have) {
         s1 = es::mk128(s1lo[i], s1hi[i]);
-        const u128 X = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
-        const u128 Y = X + len;                          // < 2M
-        const bool w = Y > M;
-        const u128 T = w ? Y - M : Y;                    // end of the run's last part, <= M
-        if (step == 0 || X > Xp) {
-          pX = mitm_lower_bound(A, X);
-        } else {
-          while (pX > 0 && ge(pX - 1, X)) --pX;
-        }
-        if (step == 0 || w != wp || T > Tp) {
-          pY = T == M ? A.n2 : mitm_lower_bound(A, T);
-        } else {
-          while (pY > 0 && ge(pY - 1, T)) --pY;
-        }
+        X = a >= s1 ? a - s1 : a + (M - s1);  // (a - s1) mod M
+        const u128 Y = X + len;               // < 2M
+        w = Y > M;
+        T = w ? Y - M : Y;                    // end of the run's last part, <= M
+        searchX = step == 0 || X > Xp;
+        searchY = step == 0 || w != wp || T > Tp;
+        if (searchX) pX = mitm_lower_bound(A, X);
+        if (searchY) pY = T == M ? A.n2 : mitm_lower_bound(A, T);
+      }
+      lower(pX, X, have && !searchX);
+      lower(pY, T, have && !searchY);
+      uint64_t st = 0, L = 0;  // run: L table entries from st, circular
+      if (have) {
         Xp = X;
         Tp = T;
         wp = w;
@@ -628,7 +642,7 @@ int es_mitm_scan(es_mitm* m, uint64_t t, uint64_t a_lo, uint64_t a_hi, uint64_t 
                                                          (long double)(1ull << 26));
   ES_HIP(hipEventRecord(m->ev0, m->stream));
   if (m->d_s1hi) {
-    // sorted sweep: 1024 consecutive s1 per wave in 64 / GL slices (GL lanes walk one s1's run:
+    // sorted sweep: 1024 consecutive s1 per wave, 64 / GL per step (GL lanes walk one s1's run:
     // 4 while runs stay below 256 entries -- more concurrent s1 per wave beat shorter walks --
     // or `walk` when it names a size); launches of ~2^34 expected candidates
     uint32_t GL = m->wave >= 4 ? m->wave : 4;

@@ -61,16 +61,18 @@ namespace {
 
 constexpr int kMaxMasks = 12;
 constexpr int kMaskInfo = 8;   // dwords per mask record
-constexpr int kSlowInfo = 8;   // dwords per exact-filter record
+constexpr int kSlowInfo = 16;  // dwords per exact-filter record
 
 // Read only by the exact tail (rare), through one pointer.
 struct ColdPlan {
   uint64_t M0_lo, M0_hi;
   uint32_t q0, r0, n_slow, stride;
+  uint32_t slow_fp, pad;      // 1: every exact-filter modulus < 2^18 (FP64 residues are exact)
   const uint32_t* strip;      // strip rows; L_sigma at dwords 2..5
   const uint32_t* e0;         // [r0] ring-0 e values, ascending
   const uint32_t* inner_ids;  // [r0] original ring-0 digit of each sorted slot
-  const uint32_t* slow;       // [n_slow][8] q, word offset, 2^32, 2^64, 2^96 mod q, t*M mod q
+  const uint32_t* slow;       // [n_slow][16] q, word offset, 2^32, 2^64, 2^96 mod q, t*M mod q,
+                              // 2 unused, then 2^32, 2^64, 2^96 mod q and 1/q as doubles
   const uint32_t* slow_bits;  // admissibility bitmaps
   unsigned long long* out;    // survivor IDs
   unsigned long long* out_n;  // survivor counter (may pass out_cap)
@@ -108,6 +110,7 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
   u128 l = (M0 - z) + L;  // l_u + L_sigma
   if (kap) l -= M0;
   const uint32_t q0 = C->q0, r0 = C->r0, ns = C->n_slow;
+  const bool fp = C->slow_fp != 0;
   while (bits) {
     const uint32_t i = 32u * w + (uint32_t)__builtin_ctz(bits);
     bits &= bits - 1u;
@@ -117,12 +120,27 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
     if (hh >= q0) hh -= q0;
     const u128 s = (u128)hh * M0 + l;
     const uint64_t s_lo = (uint64_t)s, s_hi = (uint64_t)(s >> 64);
+    const double d0 = (double)(uint32_t)s_lo, d1 = (double)(uint32_t)(s_lo >> 32),
+                 d2 = (double)(uint32_t)s_hi, d3 = (double)(uint32_t)(s_hi >> 32);
     bool ok = true;
     for (uint32_t j = 0; j < ns && ok; ++j) {
-      const uint32_t* f = C->slow + kSlowInfo * j;
+      cu32_t* f = (cu32_t*)C->slow + kSlowInfo * j;  // same j in every active lane: scalar
       const uint32_t q = f[0];
-      uint32_t r = es_mod128(s_lo, s_hi, q, f[2], f[3], f[4]) + f[5];
-      if (r >= q) r -= q;
+      uint32_t r;
+      if (fp) {
+        // every term below 2^50 and the sum below 2^53: exact; the quotient from one multiply
+        // by 1/q is off by at most one either way
+        cf64_t* fd = reinterpret_cast<cf64_t*>(f + 8);
+        const double qd = (double)q;
+        const double acc = fma(d3, fd[2], fma(d2, fd[1], fma(d1, fd[0], d0 + (double)f[5])));
+        double x = fma(-floor(acc * fd[3]), qd, acc);
+        x = x < 0.0 ? x + qd : x;
+        x = x >= qd ? x - qd : x;
+        r = (uint32_t)x;
+      } else {
+        r = es_mod128(s_lo, s_hi, q, f[2], f[3], f[4]) + f[5];
+        if (r >= q) r -= q;
+      }
       ok = (C->slow_bits[f[1] + (r >> 5)] >> (r & 31u)) & 1u;
     }
     if (ok) {
@@ -861,7 +879,11 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     r[2] = mod_u128((u128)1 << 32, q);
     r[3] = mod_u128((u128)1 << 64, q);
     r[4] = mod_u128((u128)1 << 96, q);
+    const double fd[4] = {(double)r[2], (double)r[3], (double)r[4], 1.0 / (double)q};
+    std::memcpy(r + 8, fd, sizeof(fd));
   }
+  bool slow_fp = true;
+  for (uint32_t j = 0; j < ns; ++j) slow_fp = slow_fp && d->filt_q[e->slow_order[j]] < (1u << 18);
   std::vector<uint32_t> fbits(d->filt_bits, d->filt_bits + d->n_filt_words);
   if (fbits.empty()) fbits.push_back(0);
 
@@ -900,6 +922,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   cold.r0 = r0;
   cold.n_slow = ns;
   cold.stride = stride;
+  cold.slow_fp = slow_fp ? 1u : 0u;
   cold.strip = (const uint32_t*)(base + o_strip);
   cold.e0 = (const uint32_t*)(base + o_e0);
   cold.inner_ids = (const uint32_t*)(base + o_ids);

@@ -232,8 +232,10 @@ __global__ void __launch_bounds__(256) es_mitm_kernel(MitmArgs A, uint64_t i0, u
 // s1 grows, (a - s1) mod M and (a + len - s1) mod M fall, so a group's two run ends move down
 // the table by short steps, which its GL lanes take GL entries at a time; a binary search is
 // needed only at the chunk's first step and where an end wraps past 0 or M.
+constexpr int kWalkUnroll = 2;
+
 template <int GL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
                      const uint64_t* __restrict__ s1lo, uint64_t n1, uint64_t chunk,
                      uint64_t c_lo, uint64_t c_hi) {
@@ -313,31 +315,48 @@ es_mitm_sweep_kernel(MitmArgs A, const uint64_t* __restrict__ s1hi,
         st = pX;
         L = w ? (A.n2 - pX) + pY : pY - pX;
       }
-      for (uint64_t k0 = 0; __ballot(k0 < L) != 0; k0 += GL) {  // wave-uniform
-        const uint64_t k = k0 + gl;
-        const bool ok = k < L;
-        u128 s = 0;
-        if (ok) {
-          uint64_t u = st + k;
-          if (u >= A.n2) u -= A.n2;
-          s = s1 + es::mk128(A.s2lo[u], A.s2hi[u]);
-          if (s >= M) s -= M;
-          ++tested;
+      // kWalkUnroll rounds of the walk per trip: every round's table loads are issued before
+      // the first round is tested, so their latencies overlap
+      constexpr int U = kWalkUnroll;
+      for (uint64_t k0 = 0; __ballot(k0 < L) != 0; k0 += (uint64_t)GL * U) {  // wave-uniform
+        uint64_t tl[U], th[U];
+        bool okv[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+          const uint64_t k = k0 + (uint64_t)r * GL + gl;
+          okv[r] = k < L;
+          tl[r] = th[r] = 0;
+          if (okv[r]) {
+            uint64_t u = st + k;
+            if (u >= A.n2) u -= A.n2;
+            tl[r] = A.s2lo[u];
+            th[r] = A.s2hi[u];
+          }
         }
-        if (!fp) {
-          if (ok && mitm_test(A, s)) mitm_emit(A, s);
-          continue;
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+          const bool ok = okv[r];
+          u128 s = 0;
+          if (ok) {
+            s = s1 + es::mk128(tl[r], th[r]);
+            if (s >= M) s -= M;
+            ++tested;
+          }
+          if (!fp) {
+            if (ok && mitm_test(A, s)) mitm_emit(A, s);
+            continue;
+          }
+          const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
+          const unsigned long long mask = __ballot(alive);
+          const uint32_t n = __popcll(mask);
+          if (cnt + n > 64u) flush();
+          if (alive) {
+            const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+            qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
+                                  (uint32_t)(s >> 96));
+          }
+          cnt += n;
         }
-        const bool alive = mitm_test_fp<0>(A, s, ok, 0, A.n_first);
-        const unsigned long long mask = __ballot(alive);
-        const uint32_t n = __popcll(mask);
-        if (cnt + n > 64u) flush();
-        if (alive) {
-          const uint32_t slot = cnt + __popcll(mask & ((1ull << lane) - 1ull));
-          qv[slot] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)(s >> 64),
-                                (uint32_t)(s >> 96));
-        }
-        cnt += n;
       }
     }
   }

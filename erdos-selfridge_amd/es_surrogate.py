@@ -30,6 +30,7 @@ import time
 import numpy as np
 
 import es_plan as EP
+from gpu_sieve_mi355x import AUTO_MASK_STOP
 
 N_FEAT = 64
 N_RING_FEAT = 8      # rings described by (log2 q, log2 radix)
@@ -200,7 +201,10 @@ class PlanSpace:
           16-23  kernel geometry: inner radix (ring 0, or its lowest digit group when wider
                  than 128), words per row, split flag, strip length S (leading radices nearest
                  1024), rings decoded per strip, log2 of the exact-tail fraction left by the 8
-                 best word masks, ring count, log2 of a per-position VALU estimate;
+                 best word masks, ring count, log2 of a per-position cost estimate of the
+                 kernel (its LDS row reads, W x (masks + 1) per outer index, with the mask
+                 count the engine's "auto" rule takes, plus the queued candidates the masks
+                 leave, over the inner radix);
           24-31  exponent fraction T_p / t_p of the 8 smallest primes;
           32-39  the engine's split alternatives for ring 0 (whole, digit groups <= 64, 32,
                  16): log2 inner radix and words per row of each (the engine times them and
@@ -247,7 +251,12 @@ class PlanSpace:
         F[:, o + 4] = np.maximum(nk - m, 0) / 16.0
         F[:, o + 5] = tail / -32.0
         F[:, o + 6] = nr / 20.0
-        F[:, o + 7] = np.log2((11.0 + 7.0 * N_LEAD * W) / r0k) / 8.0
+        # masks the "auto" rule stages: least pass rates while r0 x the rates taken >= the stop
+        pre = r0k[:, None] * np.concatenate([np.ones((n, 1)), np.cumprod(mr, 1)[:, :-1]], 1)
+        take = (pre >= AUTO_MASK_STOP) & (mr < 1.0)
+        nm = take.sum(1)
+        left = r0k * np.where(take, mr, 1.0).prod(1)   # candidate slots per outer index
+        F[:, o + 7] = np.log2((W * (nm + 1.0) + 20.0 * left) / r0k) / 8.0
         o += 8
         F[:, o:o + self.small] = Ei[:, :self.small] / self.tp[None, :self.small]
         o += N_SMALL

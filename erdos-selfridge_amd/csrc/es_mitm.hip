@@ -534,8 +534,11 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
   ES_HIP(hipMemcpy(d_t2hi, d->term2_hi, t2 * 8, hipMemcpyHostToDevice));
   if (m->nf)
     ES_HIP(hipMemcpy(m->d_fbits, d->filt_bits, (size_t)d->n_filt_words * 4, hipMemcpyHostToDevice));
-  // the half-2 table (and, for the sweep, the half-1 table): decode, then sort by low word and
-  // (stably) by high word; sort scratch is freed once the table is built
+  // the half-2 table (and, for the sweep, the half-1 table): decode into the table, then sort
+  // by low word into scratch and (stably) by high word back into the table; each pass sorts
+  // only the bits a value below M can have; sort scratch is freed once the table is built
+  const uint32_t hi_bits = d->M_hi ? 64u - (uint32_t)__builtin_clzll(d->M_hi) : 0u;
+  const uint32_t lo_bits = d->M_hi ? 64u : 64u - (uint32_t)__builtin_clzll(d->M_lo | 1u);
   auto build_table = [&](uint32_t w, const uint32_t* d_radix, const uint32_t* d_off,
                          const uint64_t* d_tlo, const uint64_t* d_thi, uint64_t n,
                          uint64_t* out_hi, uint64_t* out_lo) -> int {
@@ -549,31 +552,34 @@ int es_mitm_create(const es_mitm_desc* d, es_mitm** out) {
       }
       const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, (uint64_t)m->cus * 64);
       hipLaunchKernelGGL(es_mitm_table_kernel, dim3((uint32_t)blocks), dim3(256), 0, m->stream,
-                         w, d_radix, d_off, d_tlo, d_thi, d->M_lo, d->M_hi, n, k_hi, k_lo);
+                         w, d_radix, d_off, d_tlo, d_thi, d->M_lo, d->M_hi, n, out_hi, out_lo);
       if (hipGetLastError() != hipSuccess) { r = fail(ES_ERR_DEVICE, "table kernel launch failed"); break; }
       size_t tmp_bytes = 0;
-      if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, k_lo, out_lo, k_hi, out_hi, (uint32_t)n,
+      if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, out_lo, k_lo, out_hi, k_hi, (uint32_t)n,
                                     0, 64, m->stream) != hipSuccess ||
           hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)) != hipSuccess) {
         r = fail(ES_ERR_DEVICE, "radix sort set-up failed");
         break;
       }
       // low word first, then the high word (LSD radix sort is stable): 128-bit order
-      if (rocprim::radix_sort_pairs(tmp, tmp_bytes, k_lo, out_lo, k_hi, out_hi, (uint32_t)n, 0,
-                                    64, m->stream) != hipSuccess ||
-          rocprim::radix_sort_pairs(tmp, tmp_bytes, out_hi, k_hi, out_lo, k_lo, (uint32_t)n, 0,
-                                    64, m->stream) != hipSuccess ||
-          hipMemcpyAsync(out_hi, k_hi, n * 8, hipMemcpyDeviceToDevice, m->stream) != hipSuccess ||
-          hipMemcpyAsync(out_lo, k_lo, n * 8, hipMemcpyDeviceToDevice, m->stream) != hipSuccess ||
-          hipStreamSynchronize(m->stream) != hipSuccess) {
+      bool ok = rocprim::radix_sort_pairs(tmp, tmp_bytes, out_lo, k_lo, out_hi, k_hi,
+                                          (uint32_t)n, 0, lo_bits, m->stream) == hipSuccess;
+      if (ok && hi_bits) {
+        ok = rocprim::radix_sort_pairs(tmp, tmp_bytes, k_hi, out_hi, k_lo, out_lo, (uint32_t)n, 0,
+                                       hi_bits, m->stream) == hipSuccess;
+      } else if (ok) {  // every high word is 0: the low-word order is the table
+        ok = hipMemcpyAsync(out_hi, k_hi, n * 8, hipMemcpyDeviceToDevice, m->stream) == hipSuccess &&
+             hipMemcpyAsync(out_lo, k_lo, n * 8, hipMemcpyDeviceToDevice, m->stream) == hipSuccess;
+      }
+      if (!ok || hipStreamSynchronize(m->stream) != hipSuccess) {
         r = fail(ES_ERR_DEVICE, "radix sort failed");
         break;
       }
     } while (false);
-    hipStreamSynchronize(m->stream);
-    hipFree(k_hi);
-    hipFree(k_lo);
-    hipFree(tmp);
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(k_hi);
+    (void)hipFree(k_lo);
+    (void)hipFree(tmp);
     return r;
   };
   if ((rc = build_table(d->n2, d_radix2, d_off2, d_t2lo, d_t2hi, m->n2, m->d_s2hi, m->d_s2lo)))

@@ -63,7 +63,7 @@ constexpr int kMaxMasks = 12;
 constexpr int kMaskInfo = 8;   // dwords per mask record
 constexpr int kSlowInfo = 16;  // dwords per exact-filter record
 
-// Read only by the exact tail (rare), through one pointer.
+// Read only by the exact re-test of queued candidates, through one pointer.
 struct ColdPlan {
   uint64_t M0_lo, M0_hi;
   uint32_t q0, r0, n_slow, stride;
@@ -91,7 +91,7 @@ struct HotPlan {
   const uint32_t* term_h;          // base-ring terms, h part
   const uint64_t* term_l;          // base-ring terms, l part (lo, hi pairs)
   const uint32_t* masks;           // [F][8] m, table byte base, M0 mod m, 2^32, 2^64, 2^96 mod m,
-                                   //        (-M) mod m, t*M mod m
+                                   //        row multiplier (-M)^-1 mod m, t*M mod m
   const uint32_t* gmask;           // mask tables served from global memory (through L1)
   const ColdPlan* cold;
 };

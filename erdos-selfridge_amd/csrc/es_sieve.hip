@@ -97,8 +97,12 @@ struct HotPlan {
 };
 
 // Exact re-test of the candidate slots in `bits` (sorted ring-0 slots 32*w + b) of outer
-// index o; h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.
-__device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint32_t bits,
+// index o; h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.  Kernels with
+// one or two words per row inline it into the queue drain (an out-of-line call makes the whole
+// kernel allocate registers around the call: 96 VGPRs and spills against 79 and none inline
+// for 6 masks, one word); wider rows, whose unrolled word loop has one drain per word, call
+// it out of line to keep the code small.
+__device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
                                            uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
                                            uint64_t zh, uint32_t sg, uint64_t o,
                                            uint64_t idx_lo, uint64_t idx_hi) {
@@ -150,6 +154,13 @@ __device__ __noinline__ void es_exact_tail(const ColdPlan* __restrict__ C, uint3
   }
 }
 
+__device__ __noinline__ void es_exact_test_call(const ColdPlan* __restrict__ C, uint32_t bits,
+                                                uint32_t w, uint32_t h, uint32_t kap,
+                                                uint64_t zl, uint64_t zh, uint32_t sg,
+                                                uint64_t o, uint64_t idx_lo, uint64_t idx_hi) {
+  es_exact_test(C, bits, w, h, kap, zl, zh, sg, o, idx_lo, idx_hi);
+}
+
 // Per-wave candidate queue in LDS, after the tables (kWaveQueueBytes per wave): up to
 // kQueueCap entries {bits, (sg * words + w) | lane << 26} of mask words that still have
 // candidates.  Entries wait until 64 are queued and are then re-tested exactly, one per lane,
@@ -196,7 +207,10 @@ __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint3
     const bool kap = es_ge128(Lh, Ll, szh, szl);
     uint32_t h = shu + row[0] + (kap ? 1u : 0u);
     if (h >= P.q0) h -= P.q0;
-    es_exact_tail(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
+    if (W == 1 || W == 2)
+      es_exact_test(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
+    else
+      es_exact_test_call(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
   }
   __builtin_amdgcn_wave_barrier();
 }

@@ -136,21 +136,20 @@ def main() -> int:
     kern_rate = dev_pos / (dev_ms / 1e3)
     F = eng.info.n_masks_used
     # Profiled figures of es_sieve_kernel<6, 0, 1> on this wheel (rocprofv3, 2^36-position
-    # launches): 30.3 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
-    # index = r0 positions per lane), LDS busy 91 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
-    # 57 % bank-conflict stalls), HBM traffic 14.3 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
-    # copy-calibrated; the writes are the exact test's call spills).  Other instances are not
-    # profiled.
+    # launches): 30.7 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
+    # index = r0 positions per lane), LDS busy 92 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
+    # 58 % bank-conflict stalls), HBM traffic 1.47 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
+    # copy-calibrated).  Other instances are not profiled.
     profiled = F == 6 and eng.info.words == 1
-    ops_kernel = 30.3 / eng.r0 if profiled else None
+    ops_kernel = 30.7 / eng.r0 if profiled else None
     achieved = ops_ref * kern_rate
     roofline = {"bound": "valu-int32", "achieved": achieved / 1e12, "peak": INT32_PEAK / 1e12,
                 "unit": "TOP/s", "frac": achieved / INT32_PEAK,
-                "traffic": 1.43e7 if profiled else None,
+                "traffic": 1.47e6 if profiled else None,
                 "ops_per_position_reference": round(ops_ref, 3),
                 "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
                 "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
-                "lds_busy_frac_profiled": 0.91 if profiled else None,
+                "lds_busy_frac_profiled": 0.92 if profiled else None,
                 "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
                 "kernel_positions_per_s": kern_rate}
 

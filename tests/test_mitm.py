@@ -39,7 +39,7 @@ def test_split_and_wheel_choice(k, mem):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("walk,sweep", [(1, 1), (2, 1), (0, 2), (8, 2), (16, 2), (64, 2)])
+@pytest.mark.parametrize("walk,sweep", [(1, 1), (2, 1), (0, 2), (8, 2), (16, 2), (32, 2), (64, 2)])
 def test_windows_equal_range_oracle(walk, sweep):
     """walk 1 / 2: lane or wave per s1 with binary searches; sweep 2: both halves sorted, two
     run pointers per group of `walk` lanes (0: by run length) moving down the table."""

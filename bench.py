@@ -150,6 +150,8 @@ def main() -> int:
                 "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
                 "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
                 "lds_busy_frac_profiled": 0.92 if profiled else None,
+                "limiter_profiled": ("LDS: random mask-row reads, 58 % of cycles bank-conflict "
+                                     "stalls" if profiled else None),
                 "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
                 "kernel_positions_per_s": kern_rate}
 

@@ -514,6 +514,8 @@ struct es_engine {
   std::vector<u128> terms;                      // all ring terms, ring-major (host idx -> s)
   std::vector<uint32_t> term_off;
   uint64_t t = 0;
+  uint64_t last_count = 0;   // survivor count read since the last launch or reset
+  bool count_valid = false;
   void* dblob = nullptr;
   unsigned long long* d_out = nullptr;
   unsigned long long* d_count = nullptr;
@@ -542,6 +544,7 @@ int upload_tconst(es_engine* e, const std::vector<uint32_t>& tc) {
 }
 
 int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
+  e->count_valid = false;
   if (o_hi <= o_lo) return ES_OK;
   uint64_t u_lo = o_lo / e->S, u_hi = (o_hi + e->S - 1) / e->S;
   uint64_t grid = (u_hi - u_lo + e->threads - 1) / e->threads;
@@ -1040,6 +1043,7 @@ int es_engine_set_block(es_engine* e, uint64_t t) {
 
 int es_engine_reset(es_engine* e) {
   if (!e) return fail(ES_ERR_INVALID, "null engine");
+  e->count_valid = false;
   ES_HIP(hipMemsetAsync(e->d_count, 0, 8, e->stream));
   return ES_OK;
 }
@@ -1075,10 +1079,16 @@ int es_engine_sync(es_engine* e) {
 
 int es_engine_count(es_engine* e, uint64_t* n) {
   if (!e || !n) return fail(ES_ERR_INVALID, "null argument");
+  if (e->count_valid) {  // nothing launched or reset since the last read
+    *n = e->last_count;
+    return ES_OK;
+  }
   unsigned long long c = 0;
   ES_HIP(hipMemcpyAsync(&c, e->d_count, 8, hipMemcpyDeviceToHost, e->stream));
   ES_HIP(hipStreamSynchronize(e->stream));
   *n = c;
+  e->last_count = c;
+  e->count_valid = true;
   return ES_OK;
 }
 

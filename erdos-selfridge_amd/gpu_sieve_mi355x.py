@@ -298,6 +298,7 @@ class _TConst:
     def set(self, arr) -> None:
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.uint32)[:len(self._e.rest_q)])
         nat.check(nat.lib().es_engine_set_tconst(self._e._h, nat.ptr32(a), len(self._e.rest_q)))
+        self._e._tconst_block = None          # constants from the caller: no block known
 
 
 # --------------------------------------------------------------------------- engine
@@ -347,6 +348,7 @@ class GpuSieve:
         self.nrest = len(self.rest_q)
         self.tconst_h = np.zeros(max(1, self.nrest), dtype=np.uint32)
         self.tconst = _TConst(self)
+        self._tconst_block = None              # block whose constants are on the device
         self.out_cap = int(P.tun["max_survivors"])
         self.out = _Out(self)
         self.out_n = _Counter(self)
@@ -427,9 +429,12 @@ class GpuSieve:
         self.launches += 1
 
     def set_block(self, t: int) -> None:
+        if self._tconst_block == t:            # already uploaded for this block
+            return
         for j, q in enumerate(self.rest_q):
             self.tconst_h[j] = ((t % q) * self.rest_Mmodq[j]) % q
         self.tconst.set(self.tconst_h)
+        self._tconst_block = t
 
     def scan_block(self, t: int) -> list[int]:
         """All positions of block t; one sync per block.  Survivor IDs are block-absolute."""

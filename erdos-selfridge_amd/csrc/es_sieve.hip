@@ -647,10 +647,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->r0 = d->ring_radix[0];
   e->M0 = M / e->q0;
   e->W = (e->r0 + 31) / 32;
-  e->threads = d->threads ? d->threads : 256;
-  if (e->threads != 64 && e->threads != 128 && e->threads != 256 && e->threads != 512 &&
-      e->threads != 1024)
-    return fail(ES_ERR_INVALID, "threads must be 64, 128, 256, 512 or 1024");
+  e->threads = d->threads;  // 0: chosen by occupancy once the tables are sized
+  if (e->threads != 0 && e->threads != 64 && e->threads != 128 && e->threads != 256 &&
+      e->threads != 512 && e->threads != 1024)
+    return fail(ES_ERR_INVALID, "threads must be 0, 64, 128, 256, 512 or 1024");
   e->out_cap = d->out_cap ? d->out_cap : (1ull << 20);
   e->radix.assign(d->ring_radix, d->ring_radix + w);
   e->term_off = term_off;
@@ -799,8 +799,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   }
   if (lds.size() > 40960) return fail(ES_ERR_LIMIT, "ring-0 tables exceed 160 KiB of LDS");
   // per workgroup: the tables, then each wave's candidate queue
-  e->lds_bytes = (uint32_t)((lds.size() * 4 + 15) & ~size_t(15)) +
-                 (e->threads / 64) * kWaveQueueBytes;
+  const uint32_t table_bytes = (uint32_t)((lds.size() * 4 + 15) & ~size_t(15));
   if ((uint64_t)e->S * W >= (1u << kQueueIndexBits))
     return fail(ES_ERR_LIMIT, "strip rows times ring-0 words exceed the queue index");
 
@@ -924,6 +923,31 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   ES_HIP(hipGetDevice(&e->device));
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
+  e->kern = pick_kernel((int)e->F, W, G);
+  if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
+  if (!e->threads) {
+    // The workgroup size that keeps the most waves resident per CU (ties: the smaller one).
+    // Every workgroup holds its own copy of the tables, so large tables favour larger
+    // workgroups; registers and the CU's wave slots bound the rest.
+    uint32_t best_waves = 0;
+    for (uint32_t t : {256u, 512u, 1024u}) {
+      const uint32_t lds_t = table_bytes + (t / 64) * kWaveQueueBytes;
+      if (lds_t > prop.sharedMemPerBlock) continue;
+      if (lds_t > 65536)
+        ES_HIP(hipFuncSetAttribute((const void*)e->kern,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)prop.sharedMemPerBlock));
+      int nb = 0;
+      ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)e->kern, (int)t,
+                                                          lds_t));
+      if ((uint32_t)nb * (t / 64) > best_waves) {
+        best_waves = (uint32_t)nb * (t / 64);
+        e->threads = t;
+      }
+    }
+    if (!e->threads) e->threads = 256;  // nothing fits: the check below reports it
+  }
+  e->lds_bytes = table_bytes + (e->threads / 64) * kWaveQueueBytes;
   if (e->lds_bytes > prop.sharedMemPerBlock)
     return fail(ES_ERR_LIMIT, "LDS tables and candidate queues exceed the device's "
                               "per-workgroup LDS (fewer threads or a smaller budget)");
@@ -975,11 +999,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   P.masks = (const uint32_t*)(base + o_mask);
   P.gmask = (const uint32_t*)(base + o_gm);
   P.cold = (const ColdPlan*)(base + o_cold);
-  e->kern = pick_kernel((int)F, W, G);
-  if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
-  if (e->lds_bytes > 65536)
+  if (e->lds_bytes > 65536)  // the device's limit, so engines sharing a kernel never lower it
     ES_HIP(hipFuncSetAttribute((const void*)e->kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)e->lds_bytes));
+                               (int)prop.sharedMemPerBlock));
   e->launch_outer = std::max<uint64_t>(1, ((uint64_t)1 << (d->launch_log2 ? d->launch_log2 : 37)) / r0);
   e->ev_pool.resize(512);
   for (auto& ev : e->ev_pool) ES_HIP(hipEventCreate(&ev));
@@ -1422,9 +1444,13 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   ES_HIP(hipStreamSynchronize(b->stream));
   for (size_t g = 0; g < runs.size(); ++g) {
     const Launch& L = runs[g];
-    if (L.lds_max > 65536)
+    if (L.lds_max > 65536) {  // the device's limit, so batches sharing a kernel never lower it
+      int dev = 0, lim = 0;
+      ES_HIP(hipGetDevice(&dev));
+      ES_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
       ES_HIP(hipFuncSetAttribute((const void*)keys[g].fn,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_max));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    }
     out.push_back({keys[g].fn, keys[g].threads, L.plan_off, L.job_off, L.n, L.lds_max, L.wgs});
   }
   b->tables_plan_bytes = plan_bytes;

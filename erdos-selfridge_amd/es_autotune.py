@@ -23,7 +23,7 @@ import gpu_sieve_mi355x as G
 
 SPACE = {
     "mi_strip_target": [64, 256, 1024, 4096, 16384],
-    "threads": [64, 128, 256, 512],
+    "threads": ["auto", 64, 128, 256, 512],
     "mi_masks": ["auto", 4, 6, 8],
     "mi_global_masks": [0, 2],
 }

@@ -38,7 +38,8 @@ TUNABLES = {
     "cost_scale": 256,         # DP cells per bit of log2(M)
     "m_cap_bits": 120,         # hard cap on log2(M)
     "max_R_bits": 62,          # residues per block must index in uint64
-    "threads": 256,            # workgroup size
+    "threads": "auto",         # workgroup size ("auto": the engine picks the size of 256,
+                               # 512, 1024 that keeps the most waves resident per CU)
     "chunk_log2": 30,          # positions per launch unit of the reference (drivers read ochunk)
     "max_survivors": 1 << 20,  # survivor buffer capacity (u64 indices)
     "fast_filters": 4,         # filters evaluated as ring-0 word masks

@@ -237,7 +237,8 @@ class _Desc:
         d.mask_bits = nat.ptr32(self.mask_bits)
         d.n_mask_words = int(self.mask_bits.size)
         d.strip_target = int(tun.get("mi_strip_target", 1024))
-        d.threads = int(tun.get("threads", 256))
+        th = tun.get("threads", "auto")
+        d.threads = 0 if th == "auto" else int(th)
         d.out_cap = int(tun.get("max_survivors", 1 << 20))
         d.lds_budget = int(tun.get("mi_lds_budget", 65536))
         d.global_masks = int(tun.get("mi_global_masks", 0))
@@ -354,7 +355,7 @@ class GpuSieve:
         self.out_n = _Counter(self)
         self.r0 = P.radix[0] if P.radix else 1
         self.n_fast = info.n_masks_used
-        self.threads = int(P.tun["threads"])
+        self.threads = int(info.threads)       # the workgroup size the engine uses
         self.chunk = 1 << int(P.tun.get("mi_launch_log2", 37))
         self.R_outer = P.R // self.r0
         # the kernel may sweep ring 0 as digit rings: one plan outer index = f kernel ones

@@ -74,7 +74,9 @@ typedef struct es_plan_desc {
   const uint32_t* mask_bits;   /* bit r set iff residue r mod m passes the mask */
   uint32_t n_mask_words;
   uint32_t strip_target;       /* target outer indices per uniform strip (0 = default) */
-  uint32_t threads;            /* workgroup size: 64, 128, 256, 512 or 1024 (0 = 256) */
+  uint32_t threads;            /* workgroup size: 64, 128, 256, 512 or 1024; 0 = the
+                                   size of 256, 512, 1024 that keeps the most waves
+                                   resident per CU (ties: the smaller) */
   uint64_t out_cap;            /* survivor buffer capacity in u64 IDs (0 = 2^20) */
   uint32_t lds_budget;         /* max bytes of LDS tables per workgroup (0 = 64 KiB) */
   uint32_t global_masks;       /* masks read through L1 instead of LDS: 0, 2 or 4 (r0 <= 32) */

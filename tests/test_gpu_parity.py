@@ -241,3 +241,22 @@ def test_masks_not_prime_to_M_are_skipped(monkeypatch):
         f"staged {eng.info.n_masks_used}")
     assert eng._d.masks[0] == extra and eng.info.n_masks_used == offered - 1
     window_check(30, None, 1, 0, min(1 << 22, plan.R))
+
+
+@pytest.mark.parametrize("k", [301, 399])
+def test_auto_workgroup_size(k):
+    """threads="auto" picks the workgroup size of 256, 512, 1024 that keeps the most waves
+    resident per CU: k=301's default wheel stages ~61 KB of two-word mask rows, which at 256
+    threads leaves two workgroups (8 waves) per CU, so a larger size must win there.  Every
+    size scans the oracle's survivors."""
+    wheel = K399_PRIMARY if k == 399 else None
+    eng, plan = engine(k, wheel)
+    say(f"case k={k} auto threads={eng.threads} lds={eng.info.lds_bytes} r0={eng.r0}")
+    assert eng.threads in (256, 512, 1024)
+    if k == 301:
+        assert eng.threads >= 512 and eng.info.lds_bytes > 60000
+    rng = random.Random(k)
+    span = min(1 << 22, plan.R // 2)
+    lo = rng.randrange(0, plan.R - span)
+    for th in ("auto", 256, 1024):
+        window_check(k, wheel, 1, lo, lo + span, limit=8, threads=th)

@@ -305,21 +305,23 @@ class _TConst:
 # --------------------------------------------------------------------------- engine
 
 def timed_rate(scan, R: int, strip_positions: int, max_s: float = 0.05,
-               strips_log2: int = 19, lo_log2: int = 28, hi_log2: int = 34) -> float:
+               strips_log2: int = 19, lo_log2: int = 28, hi_log2: int = 34,
+               floor: float = 0.0) -> float:
     """Positions per second of a layout, from scan(lo, hi) -> rate (device time) on windows
     centred in [0, R).  The target window holds 2^strips_log2 strips (about 8 workgroups per CU,
     so long strips are not timed on an under-filled GPU), clamped to [2^lo_log2, 2^hi_log2]
     positions.  Windows grow from 2^24 positions, each to what the latest rate says fits in
-    max_s, until the target is reached or the next window would not fit; the last rate is
-    returned."""
+    max_s, until the target is reached or the next window would not grow by a quarter; the
+    last rate is returned.  A layout measuring below `floor` (half the best rate so far, say)
+    after its second window stops there: it cannot win."""
     n = max(1, min(R // 2, 1 << hi_log2, max(1 << lo_log2, strip_positions << strips_log2)))
     w = max(1, min(n, 1 << 24))
     rate = scan((R - w) // 2, (R - w) // 2 + w)
-    for _ in range(8):
-        if w >= n:
+    for i in range(8):
+        if w >= n or (i >= 1 and rate < floor):
             break
         want = int(rate * max_s)
-        if want <= w:
+        if want < w + w // 4:
             break
         w = min(n, want)
         rate = scan((R - w) // 2, (R - w) // 2 + w)
@@ -416,7 +418,8 @@ class GpuSieve:
                     nat.check(L.es_engine_sync(h))
                     nat.check(L.es_engine_kernel_time(h, C.byref(ms), C.byref(la), C.byref(pos)))
                     return pos.value / max(ms.value * 1e-3, 1e-12)
-                rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05)
+                rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05,
+                                  floor=0.5 * best_rate)
                 if rate > best_rate:
                     best, best_rate = bound, rate
             finally:

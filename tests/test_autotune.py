@@ -41,6 +41,32 @@ def test_coord_keeps_default_within_margin(monkeypatch):
     assert got == {x: plan.tun[x] for x in AT.SPACE}
 
 
+def test_timed_rate_windows():
+    """Layout timing: windows grow from 2^24 positions toward the target at a fixed device
+    rate, stop once the next window would not grow by a quarter, and a layout below the
+    floor stops after its second window."""
+    import gpu_sieve_mi355x as G
+    calls = []
+
+    def scan_at(rate):
+        def scan(lo, hi):
+            calls.append(hi - lo)
+            return rate
+        return scan
+    R = 1 << 50
+    assert G.timed_rate(scan_at(1e12), R, 1 << 16, max_s=0.05) == 1e12
+    assert calls[0] == 1 << 24 and calls[-1] == 1 << 34 and len(calls) == 2
+    calls.clear()
+    G.timed_rate(scan_at(1e9), R, 1 << 16, max_s=0.05)      # 5e7 positions fit in 50 ms
+    assert calls == [1 << 24, int(1e9 * 0.05)]
+    calls.clear()
+    G.timed_rate(scan_at(1e9), R, 1 << 16, max_s=0.05, floor=2e9)
+    assert len(calls) == 2
+    calls.clear()
+    G.timed_rate(scan_at(1e9), R, 1 << 16, max_s=0.1, floor=2e9)
+    assert len(calls) == 2                                    # would grow, but below the floor
+
+
 def test_cache_roundtrip(monkeypatch, tmp_path):
     calls = []
 

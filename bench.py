@@ -450,7 +450,8 @@ def bench_k200(a) -> int:
                    "R": plan.R, "filters": len(plan.rest), "autotuned": bool(a.autotune),
                    "tune": {x: plan.tun[x] for x in ("mi_masks", "threads", "mi_strip_target")},
                    "inner_bound": eng.inner_bound,
-                   "kernel_ring0_radix": eng._d.r0, "strip_len": eng.strip_len},
+                   "kernel_ring0_radix": eng._d.r0, "strip_len": eng.strip_len,
+                   "threads": eng.threads},
         "survivors": len(ns), "survivor_hash": O.survivor_hash(ns),
         "min": str(mn) if mn is not None else None,
         "roofline": {"bound": "valu-int32", "achieved": ops_ref * kern_rate / 1e12,

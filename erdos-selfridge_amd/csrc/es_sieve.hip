@@ -84,6 +84,7 @@ struct HotPlan {
   uint64_t M0_lo, M0_hi;
   uint32_t q0, r0, S, nb, W, stride;
   uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
+  uint32_t carry_bytes, pad;       // 1: the carry table holds j(h) bytes (kernel instance C = 1)
   const uint32_t* lds_src;         // device copy of the LDS image
   const uint32_t* strip;           // [S][stride] H, 0, L(4), 0, 0, Q8_f (row byte step per f)
   const uint32_t* base_radix;      // [nb]
@@ -242,7 +243,7 @@ __device__ __forceinline__ void es_enqueue(const HotPlan& P, uint2* q, uint32_t&
 // row in table f; an outer sum that wraps past M starts one row on.  The words for slots
 // without an inner wrap and with one are ANDed over the masks first and merged by the carry
 // word once.
-template <int F, int G, int W, bool CHECK>
+template <int F, int G, int W, int C, bool CHECK>
 __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
                                          const uint32_t (&A)[F + 1], uint32_t hu, uint64_t zl,
                                          uint64_t zh, uint64_t obase, uint32_t sb, uint32_t se,
@@ -265,7 +266,13 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
     const bool live = !CHECK || (sg >= sb && sg < se);
     const uint32_t wrap = c1 ? 8u * nw : 0u;        // one row on
     if (W == 1) {
-      const uint32_t carry = carry_tab[h];
+      uint32_t carry;
+      if (C) {  // byte table of j(h): bits i >= j carry
+        const uint32_t j = reinterpret_cast<const uint8_t*>(carry_tab)[h];
+        carry = j >= 32u ? 0u : 0xffffffffu << j;
+      } else {
+        carry = carry_tab[h];
+      }
       uint32_t bx = 0xffffffffu, by = 0xffffffffu;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
@@ -328,7 +335,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
 // Strip u of a plan (u may be a replay of a valid strip for lanes past the range: `in` = false
 // makes its range empty).  Decodes the strip base over rings m+1.. in (h, l) form, sets up the
 // lane's mask-row addresses, then sweeps the strip.
-template <int F, int G, int W>
+template <int F, int G, int W, int C>
 __device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, uint64_t u,
                                          bool in, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo,
                                          uint64_t idx_hi) {
@@ -387,12 +394,12 @@ __device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, 
   }
   const bool full = in && sb == 0 && se == S;
   if (__all(full))
-    es_sweep<F, G, W, false>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
+    es_sweep<F, G, W, C, false>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
   else
-    es_sweep<F, G, W, true>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
+    es_sweep<F, G, W, C, true>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
 }
 
-template <int F, int G, int W>
+template <int F, int G, int W, int C>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, uint64_t o_hi,
                 uint64_t idx_lo, uint64_t idx_hi) {
@@ -404,7 +411,7 @@ es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, ui
   for (uint64_t base = u_lo + (uint64_t)blockIdx.x * blockDim.x; base < u_hi; base += step) {
     const uint64_t my = base + threadIdx.x;
     const bool in = my < u_hi;
-    es_strip<F, G, W>(P, lds, in ? my : u_lo, in, o_lo, o_hi, idx_lo, idx_hi);
+    es_strip<F, G, W, C>(P, lds, in ? my : u_lo, in, o_lo, o_hi, idx_lo, idx_hi);
   }
 }
 
@@ -419,7 +426,7 @@ struct GridJob {
   uint32_t plan, pad;
 };
 
-template <int F, int G, int W>
+template <int F, int G, int W, int C>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
                      uint32_t n_jobs) {
@@ -436,36 +443,38 @@ es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restric
   __syncthreads();
   const uint64_t my = J.u_lo + (b - J.wg_start) * blockDim.x + threadIdx.x;
   const bool in = my < J.u_hi;
-  es_strip<F, G, W>(P, lds, in ? my : J.u_lo, in, J.o_lo, J.o_hi, J.idx_lo, J.idx_hi);
+  es_strip<F, G, W, C>(P, lds, in ? my : J.u_lo, in, J.o_lo, J.o_hi, J.idx_lo, J.idx_hi);
 }
 
 typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
 typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
 
-// Instances: every F <= 12 with all masks in LDS for W = 1, 2 and runtime W; for W = 1 also
-// G = 2 or 4 of the masks in global memory.
+// Instances: every F <= 12 with all masks in LDS for W = 1, 2, 3, 4 and runtime W; for W = 1
+// also G = 2 or 4 of the masks in global memory, and (G = 0) the byte carry table (C = 1).
 template <int F>
-KernelFn kernel_for(uint32_t W, uint32_t G) {
+KernelFn kernel_for(uint32_t W, uint32_t G, uint32_t C) {
+  if (C) return W == 1 && G == 0 ? es_sieve_kernel<F, 0, 1, 1> : nullptr;
   if (W == 1) {
-    if (G == 2 && F >= 2) return es_sieve_kernel<F, (F >= 2 ? 2 : 0), 1>;
-    if (G == 4 && F >= 4) return es_sieve_kernel<F, (F >= 4 ? 4 : 0), 1>;
-    return G == 0 ? es_sieve_kernel<F, 0, 1> : nullptr;
+    if (G == 2 && F >= 2) return es_sieve_kernel<F, (F >= 2 ? 2 : 0), 1, 0>;
+    if (G == 4 && F >= 4) return es_sieve_kernel<F, (F >= 4 ? 4 : 0), 1, 0>;
+    return G == 0 ? es_sieve_kernel<F, 0, 1, 0> : nullptr;
   }
   if (G != 0) return nullptr;
-  if (W == 2) return es_sieve_kernel<F, 0, 2>;
-  if (W == 3) return es_sieve_kernel<F, 0, 3>;
-  if (W == 4) return es_sieve_kernel<F, 0, 4>;
-  return es_sieve_kernel<F, 0, 0>;
+  if (W == 2) return es_sieve_kernel<F, 0, 2, 0>;
+  if (W == 3) return es_sieve_kernel<F, 0, 3, 0>;
+  if (W == 4) return es_sieve_kernel<F, 0, 4, 0>;
+  return es_sieve_kernel<F, 0, 0, 0>;
 }
 
 template <int F>
-GridBatchFn grid_batch_for(uint32_t W, uint32_t G) {
+GridBatchFn grid_batch_for(uint32_t W, uint32_t G, uint32_t C) {
   if (G != 0) return nullptr;
-  if (W == 1) return es_grid_batch_kernel<F, 0, 1>;
-  if (W == 2) return es_grid_batch_kernel<F, 0, 2>;
-  if (W == 3) return es_grid_batch_kernel<F, 0, 3>;
-  if (W == 4) return es_grid_batch_kernel<F, 0, 4>;
-  return es_grid_batch_kernel<F, 0, 0>;
+  if (C) return W == 1 ? es_grid_batch_kernel<F, 0, 1, 1> : nullptr;
+  if (W == 1) return es_grid_batch_kernel<F, 0, 1, 0>;
+  if (W == 2) return es_grid_batch_kernel<F, 0, 2, 0>;
+  if (W == 3) return es_grid_batch_kernel<F, 0, 3, 0>;
+  if (W == 4) return es_grid_batch_kernel<F, 0, 4, 0>;
+  return es_grid_batch_kernel<F, 0, 0, 0>;
 }
 
 
@@ -487,8 +496,12 @@ GridBatchFn grid_batch_for(uint32_t W, uint32_t G) {
     default: return nullptr;           \
   }
 
-KernelFn pick_kernel(int F, uint32_t W, uint32_t G) { ES_F_SWITCH(kernel_for, F, W, G) }
-GridBatchFn pick_grid_batch(int F, uint32_t W, uint32_t G) { ES_F_SWITCH(grid_batch_for, F, W, G) }
+KernelFn pick_kernel(int F, uint32_t W, uint32_t G, uint32_t C) {
+  ES_F_SWITCH(kernel_for, F, W, G, C)
+}
+GridBatchFn pick_grid_batch(int F, uint32_t W, uint32_t G, uint32_t C) {
+  ES_F_SWITCH(grid_batch_for, F, W, G, C)
+}
 
 struct Blob {
   std::vector<uint8_t> bytes;
@@ -714,12 +727,16 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // a row's second word is the next row's first.  A mask whose m is not prime to M has no
   // such c^-1 and is not staged (it is only a prefilter; its filter stays exact).
   const uint32_t budget_words = (d->lds_budget ? d->lds_budget : 65536u) / 4;
-  // Carry layout.  W <= 2: one row of W words per h (bit i set iff h + e_i >= q0).  W > 2:
-  // slots are sorted by e_i, so the carrying slots of row h are a suffix [j(h), r0); the table
-  // holds j(h) as u16 (q0 entries), and the kernel builds each carry word from it.
-  const bool carry_suffix = W > 2;
-  if (carry_suffix && r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
-  const uint32_t carry_words = carry_suffix ? (q0 + 1) / 2 : q0 * W;
+  // Carry layout.  Slots are sorted by e_i, so the slots of row h with h + e_i >= q0 are a
+  // suffix [j(h), r0).  W = 1 and q0 <= 256: the table holds j(h) as one byte per h and the
+  // kernel builds the carry word from it: the table spans at most 64 dwords, one per LDS
+  // bank, so its random reads never conflict (7 % faster at k=399, q0 = 107, than word rows;
+  // with q0 > 256 the bytes conflict too and the extra instructions cost 3 %).  Otherwise W <=
+  // 2: one row of W words per h (bit i set iff j(h) <= i < r0).  W > 2: j(h) as u16, each
+  // carry word built from it (word rows would not fit).
+  const uint32_t jbytes = (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
+  if (r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
+  const uint32_t carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
   std::vector<uint32_t> staged;  // indices into the descriptor's mask list
   {
     size_t used = carry_words + 8;
@@ -786,16 +803,16 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->mask_src = msrc;
   const uint32_t carry_off = (uint32_t)((lds.size() + 3) & ~size_t(3));
   lds.resize(carry_off + carry_words, 0);
-  if (carry_suffix) {
-    for (uint32_t h = 0; h < q0; ++h) {
-      const uint32_t j = (uint32_t)(std::lower_bound(e_sorted.begin(), e_sorted.begin() + r0,
-                                                     q0 - h) - e_sorted.begin());
+  for (uint32_t h = 0; h < q0; ++h) {
+    const uint32_t j = (uint32_t)(std::lower_bound(e_sorted.begin(), e_sorted.begin() + r0,
+                                                   q0 - h) - e_sorted.begin());
+    if (jbytes == 1) {
+      lds[carry_off + h / 4] |= j << (8 * (h & 3));
+    } else if (jbytes == 2) {
       lds[carry_off + h / 2] |= j << (16 * (h & 1));
+    } else {
+      for (uint32_t i = j; i < r0; ++i) lds[carry_off + (size_t)h * W + (i >> 5)] |= 1u << (i & 31);
     }
-  } else {
-    for (uint32_t h = 0; h < q0; ++h)
-      for (uint32_t i = 0; i < r0; ++i)
-        if (h + e_sorted[i] >= q0) lds[carry_off + (size_t)h * W + (i >> 5)] |= 1u << (i & 31);
   }
   if (lds.size() > 40960) return fail(ES_ERR_LIMIT, "ring-0 tables exceed 160 KiB of LDS");
   // per workgroup: the tables, then each wave's candidate queue
@@ -923,7 +940,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   ES_HIP(hipGetDevice(&e->device));
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
-  e->kern = pick_kernel((int)e->F, W, G);
+  e->kern = pick_kernel((int)e->F, W, G, jbytes == 1 ? 1u : 0u);
   if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
   if (!e->threads) {
     // The workgroup size that keeps the most waves resident per CU (ties: the smaller one).
@@ -990,6 +1007,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   P.stride = stride;
   P.lds_words = (uint32_t)lds.size();
   P.carry_base = carry_off * 4;
+  P.carry_bytes = jbytes == 1 ? 1u : 0u;
   P.lds_src = (const uint32_t*)(base + o_lds);
   P.strip = (const uint32_t*)(base + o_strip);
   P.base_radix = (const uint32_t*)(base + o_brad);
@@ -1374,7 +1392,7 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   std::vector<Key> keys;
   for (uint32_t i = 0; i < b->jobs.size(); ++i) {
     const es_engine* e = b->jobs[i].e;
-    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0);
+    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0, e->hp.carry_bytes);
     if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
     const uint32_t occ = std::min<uint32_t>(2048u / e->threads,
                                             163840u / std::max<uint32_t>(e->lds_bytes, 1));

@@ -135,13 +135,13 @@ def main() -> int:
     ops_ref = 12 + 10 * ef                              # reference-algorithm ops per position
     kern_rate = dev_pos / (dev_ms / 1e3)
     F = eng.info.n_masks_used
-    # Profiled figures of es_sieve_kernel<6, 0, 1> on this wheel (rocprofv3, 2^36-position
-    # launches): 30.7 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
-    # index = r0 positions per lane), LDS busy 92 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
-    # 58 % bank-conflict stalls), HBM traffic 1.47 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
+    # Profiled figures of es_sieve_kernel<6, 0, 1, 1> on this wheel (rocprofv3, 2^36-position
+    # launches): 33.6 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
+    # index = r0 positions per lane), LDS busy 89 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
+    # 54 % bank-conflict stalls), HBM traffic 1.47 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
     # copy-calibrated).  Other instances are not profiled.
     profiled = F == 6 and eng.info.words == 1
-    ops_kernel = 30.7 / eng.r0 if profiled else None
+    ops_kernel = 33.6 / eng.r0 if profiled else None
     achieved = ops_ref * kern_rate
     roofline = {"bound": "valu-int32", "achieved": achieved / 1e12, "peak": INT32_PEAK / 1e12,
                 "unit": "TOP/s", "frac": achieved / INT32_PEAK,
@@ -149,8 +149,8 @@ def main() -> int:
                 "ops_per_position_reference": round(ops_ref, 3),
                 "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
                 "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
-                "lds_busy_frac_profiled": 0.92 if profiled else None,
-                "limiter_profiled": ("LDS: random mask-row reads, 58 % of cycles bank-conflict "
+                "lds_busy_frac_profiled": 0.89 if profiled else None,
+                "limiter_profiled": ("LDS: random mask-row reads, 54 % of cycles bank-conflict "
                                      "stalls" if profiled else None),
                 "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
                 "kernel_positions_per_s": kern_rate}

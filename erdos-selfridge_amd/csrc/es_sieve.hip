@@ -70,7 +70,7 @@ struct ColdPlan {
   uint32_t slow_fp, pad;      // 1: every exact-filter modulus < 2^18 (FP64 residues are exact)
   const uint32_t* strip;      // strip rows; L_sigma at dwords 2..5
   const uint32_t* e0;         // [r0] ring-0 e values, ascending
-  const uint32_t* inner_ids;  // [r0] original ring-0 digit of each sorted slot
+  const uint32_t* inner_ids;  // [r0] idx offset of each sorted slot: its ring-0 digit x lead
   const uint32_t* slow;       // [n_slow][16] q, word offset, 2^32, 2^64, 2^96 mod q, t*M mod q,
                               // 2 unused, then 2^32, 2^64, 2^96 mod q and 1/q as doubles
   const uint32_t* slow_bits;  // admissibility bitmaps
@@ -84,7 +84,7 @@ struct HotPlan {
   uint64_t M0_lo, M0_hi;
   uint32_t q0, r0, S, nb, W, stride;
   uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
-  uint32_t carry_bytes, pad;       // 1: the carry table holds j(h) bytes (kernel instance C = 1)
+  uint32_t carry_bytes, pad;       // 1: the carry table holds j(h) bytes (kernel instance C & 1)
   const uint32_t* lds_src;         // device copy of the LDS image
   const uint32_t* strip;           // [S][stride] H, 0, L(4), 0, 0, Q8_f (row byte step per f)
   const uint32_t* base_radix;      // [nb]
@@ -98,11 +98,14 @@ struct HotPlan {
 };
 
 // Exact re-test of the candidate slots in `bits` (sorted ring-0 slots 32*w + b) of outer
-// index o; h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.  Kernels with
+// index o (strip row sg); h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.
+// C & 2: the kernel's rings 0 and 1 are the reference's rings 1 and 0; a slot's idx is then
+// o * r0 + its offset + the strip row's signed correction (dword 1 of the row).  Kernels with
 // one or two words per row inline it into the queue drain (an out-of-line call makes the whole
 // kernel allocate registers around the call: 96 VGPRs and spills against 79 and none inline
 // for 6 masks, one word); wider rows, whose unrolled word loop has one drain per word, call
 // it out of line to keep the code small.
+template <int CF>
 __device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
                                            uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
                                            uint64_t zh, uint32_t sg, uint64_t o,
@@ -119,7 +122,8 @@ __device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, ui
   while (bits) {
     const uint32_t i = 32u * w + (uint32_t)__builtin_ctz(bits);
     bits &= bits - 1u;
-    const uint64_t idx = o * r0 + C->inner_ids[i];
+    uint64_t idx = o * r0 + C->inner_ids[i];
+    if (CF & 2) idx += (uint64_t)(int64_t)(int32_t)row[1];
     if (idx < idx_lo || idx >= idx_hi) continue;
     uint32_t hh = h + C->e0[i];
     if (hh >= q0) hh -= q0;
@@ -159,7 +163,7 @@ __device__ __noinline__ void es_exact_test_call(const ColdPlan* __restrict__ C, 
                                                 uint32_t w, uint32_t h, uint32_t kap,
                                                 uint64_t zl, uint64_t zh, uint32_t sg,
                                                 uint64_t o, uint64_t idx_lo, uint64_t idx_hi) {
-  es_exact_test(C, bits, w, h, kap, zl, zh, sg, o, idx_lo, idx_hi);
+  es_exact_test<0>(C, bits, w, h, kap, zl, zh, sg, o, idx_lo, idx_hi);
 }
 
 // Per-wave candidate queue in LDS, after the tables (kWaveQueueBytes per wave): up to
@@ -183,7 +187,7 @@ __device__ __forceinline__ uint32_t es_from_lane(uint32_t src, uint32_t v) {
 
 // Exact re-test of queued entries [first, first + n), one per lane.  Called by the whole wave
 // (the permutes read every lane's strip data).
-template <int W>
+template <int W, int C>
 __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint32_t first,
                                          uint32_t n, uint32_t hu, uint64_t zl, uint64_t zh,
                                          uint64_t obase, uint64_t idx_lo, uint64_t idx_hi) {
@@ -209,7 +213,7 @@ __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint3
     uint32_t h = shu + row[0] + (kap ? 1u : 0u);
     if (h >= P.q0) h -= P.q0;
     if (W == 1 || W == 2)
-      es_exact_test(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
+      es_exact_test<C>(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
     else
       es_exact_test_call(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
   }
@@ -218,7 +222,7 @@ __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint3
 
 // Queues this lane's candidate word (bits != 0); drains 64 entries once that many wait.
 // `cnt` is wave-uniform.
-template <int W>
+template <int W, int C>
 __device__ __forceinline__ void es_enqueue(const HotPlan& P, uint2* q, uint32_t& cnt,
                                            uint32_t bits, uint32_t sw, uint32_t hu, uint64_t zl,
                                            uint64_t zh, uint64_t obase, uint64_t idx_lo,
@@ -233,7 +237,7 @@ __device__ __forceinline__ void es_enqueue(const HotPlan& P, uint2* q, uint32_t&
   cnt += (uint32_t)__popcll(any);
   if (cnt >= 64u) {
     cnt -= 64u;
-    es_drain<W>(P, q, cnt, 64u, hu, zl, zh, obase, idx_lo, idx_hi);
+    es_drain<W, C>(P, q, cnt, 64u, hu, zl, zh, obase, idx_lo, idx_hi);
   }
 }
 
@@ -267,7 +271,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
     const uint32_t wrap = c1 ? 8u * nw : 0u;        // one row on
     if (W == 1) {
       uint32_t carry;
-      if (C) {  // byte table of j(h): bits i >= j carry
+      if (C & 1) {  // byte table of j(h): bits i >= j carry
         const uint32_t j = reinterpret_cast<const uint8_t*>(carry_tab)[h];
         carry = j >= 32u ? 0u : 0xffffffffu << j;
       } else {
@@ -285,7 +289,7 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
       uint32_t bits = (carry & by) | (~carry & bx);
       if (F == 0) bits = P.r0 >= 32 ? 0xffffffffu : ((1u << P.r0) - 1u);
       if (!live) bits = 0;
-      es_enqueue<W>(P, wq, cnt, bits, sg, hu, zl, zh, obase, idx_lo, idx_hi);
+      es_enqueue<W, C>(P, wq, cnt, bits, sg, hu, zl, zh, obase, idx_lo, idx_hi);
     } else if (W == 2) {
       const uint2 carry = *reinterpret_cast<const uint2*>(carry_tab + 2 * h);
       uint32_t b0x = 0xffffffffu, b0y = 0xffffffffu, b1x = 0xffffffffu, b1y = 0xffffffffu;
@@ -302,8 +306,8 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
       uint32_t b1 = (carry.y & b1y) | (~carry.y & b1x);
       if (F == 0) b1 = P.r0 >= 64 ? 0xffffffffu : ((1u << (P.r0 - 32)) - 1u);
       if (!live) b0 = b1 = 0;
-      es_enqueue<W>(P, wq, cnt, b0, 2 * sg, hu, zl, zh, obase, idx_lo, idx_hi);
-      es_enqueue<W>(P, wq, cnt, b1, 2 * sg + 1, hu, zl, zh, obase, idx_lo, idx_hi);
+      es_enqueue<W, C>(P, wq, cnt, b0, 2 * sg, hu, zl, zh, obase, idx_lo, idx_hi);
+      es_enqueue<W, C>(P, wq, cnt, b1, 2 * sg + 1, hu, zl, zh, obase, idx_lo, idx_hi);
     } else {
       // carrying slots are the suffix [j, r0) of the sorted ring-0 slots
       const uint32_t j = reinterpret_cast<const uint16_t*>(carry_tab)[h];
@@ -325,11 +329,11 @@ __device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
           bits = n >= 32u ? 0xffffffffu : ((1u << n) - 1u);
         }
         if (!live) bits = 0;
-        es_enqueue<W>(P, wq, cnt, bits, sg * nw + w, hu, zl, zh, obase, idx_lo, idx_hi);
+        es_enqueue<W, C>(P, wq, cnt, bits, sg * nw + w, hu, zl, zh, obase, idx_lo, idx_hi);
       }
     }
   }
-  if (cnt) es_drain<W>(P, wq, 0, cnt, hu, zl, zh, obase, idx_lo, idx_hi);
+  if (cnt) es_drain<W, C>(P, wq, 0, cnt, hu, zl, zh, obase, idx_lo, idx_hi);
 }
 
 // Strip u of a plan (u may be a replay of a valid strip for lanes past the range: `in` = false
@@ -450,10 +454,16 @@ typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64
 typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
 
 // Instances: every F <= 12 with all masks in LDS for W = 1, 2, 3, 4 and runtime W; for W = 1
-// also G = 2 or 4 of the masks in global memory, and (G = 0) the byte carry table (C = 1).
+// also G = 2 or 4 of the masks in global memory, and (G = 0) the byte carry table (C & 1)
+// and the swapped lead rings' idx form (C & 2).
 template <int F>
 KernelFn kernel_for(uint32_t W, uint32_t G, uint32_t C) {
-  if (C) return W == 1 && G == 0 ? es_sieve_kernel<F, 0, 1, 1> : nullptr;
+  if (C) {
+    if (W != 1 || G != 0) return nullptr;
+    if (C == 1) return es_sieve_kernel<F, 0, 1, 1>;
+    if (C == 2) return es_sieve_kernel<F, 0, 1, 2>;
+    return C == 3 ? es_sieve_kernel<F, 0, 1, 3> : nullptr;
+  }
   if (W == 1) {
     if (G == 2 && F >= 2) return es_sieve_kernel<F, (F >= 2 ? 2 : 0), 1, 0>;
     if (G == 4 && F >= 4) return es_sieve_kernel<F, (F >= 4 ? 4 : 0), 1, 0>;
@@ -469,7 +479,12 @@ KernelFn kernel_for(uint32_t W, uint32_t G, uint32_t C) {
 template <int F>
 GridBatchFn grid_batch_for(uint32_t W, uint32_t G, uint32_t C) {
   if (G != 0) return nullptr;
-  if (C) return W == 1 ? es_grid_batch_kernel<F, 0, 1, 1> : nullptr;
+  if (C) {
+    if (W != 1) return nullptr;
+    if (C == 1) return es_grid_batch_kernel<F, 0, 1, 1>;
+    if (C == 2) return es_grid_batch_kernel<F, 0, 1, 2>;
+    return C == 3 ? es_grid_batch_kernel<F, 0, 1, 3> : nullptr;
+  }
   if (W == 1) return es_grid_batch_kernel<F, 0, 1, 0>;
   if (W == 2) return es_grid_batch_kernel<F, 0, 2, 0>;
   if (W == 3) return es_grid_batch_kernel<F, 0, 3, 0>;
@@ -522,6 +537,9 @@ struct es_engine {
   uint32_t n_filters = 0, n_slow = 0, lds_bytes = 0, max_grid = 1;
   u128 M = 1, M0 = 1, R = 1;
   uint64_t R_outer = 1, strips = 1, out_cap = 1u << 20, launch_outer = 1;
+  uint32_t lead = 1;   // > 1: rings 0 and 1 swapped (the reference's ring-0 radix)
+  uint32_t unit = 1;   // positions per es_engine_scan outer unit: lead if swapped, else r0
+  uint32_t kind = 0;   // kernel instance C: 1 byte carry table, 2 swapped lead rings' idx form
   std::vector<uint32_t> filt_q, slow_order, mask_mod, mask_src, radix;
   std::vector<uint32_t> mask_info, slow_info;  // host copies of the device records
   std::vector<u128> terms;                      // all ring terms, ring-major (host idx -> s)
@@ -556,6 +574,16 @@ int upload_tconst(es_engine* e, const std::vector<uint32_t>& tc) {
   return ES_OK;
 }
 
+// Kernel outer indices [o_lo, o_hi) whose positions include every idx in [idx_lo, idx_hi):
+// idx / r0 in the reference order; with swapped lead rings, the whole runs of lead * r0 idx
+// (lead outer indices each) that meet the window.  The exact test drops positions outside it.
+void outer_range(const es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t* o_lo,
+                 uint64_t* o_hi) {
+  const uint64_t per = (uint64_t)e->lead * e->r0;
+  *o_lo = idx_lo / per * e->lead;
+  *o_hi = idx_hi > idx_lo ? ((idx_hi - 1) / per + 1) * e->lead : *o_lo;
+}
+
 int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
   e->count_valid = false;
   if (o_hi <= o_lo) return ES_OK;
@@ -576,7 +604,10 @@ int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, ui
   ES_HIP(hipGetLastError());
   if (b) {
     ES_HIP(hipEventRecord(b, e->stream));
-    const uint64_t lo = std::max(idx_lo, o_lo * e->r0), hi = std::min(idx_hi, o_hi * e->r0);
+    // idx covered by outer indices [o_lo, o_hi) (multiples of lead when rings are swapped)
+    const uint64_t per = (uint64_t)e->lead * e->r0;
+    const uint64_t lo = std::max(idx_lo, o_lo / e->lead * per),
+                   hi = std::min(idx_hi, o_hi / e->lead * per);
     e->positions += hi > lo ? hi - lo : 0;
   }
   e->launches += 1;
@@ -658,6 +689,11 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   if (e->q0 % d->ring_mod[0] != 0 || M % e->q0 != 0 || e->q0 >= (1u << 31))
     return fail(ES_ERR_INVALID, "inner_mod must divide M, be a multiple of ring_mod[0] and < 2^31");
   e->r0 = d->ring_radix[0];
+  e->lead = d->lead_radix > 1 ? d->lead_radix : 1;
+  if (e->lead > 1 && (w < 2 || d->ring_radix[1] != e->lead || d->inner_mod != 0))
+    return fail(ES_ERR_INVALID, "lead_radix needs two rings, ring_radix[1] == lead_radix and no "
+                                "digit split of ring 0");
+  e->unit = e->lead > 1 ? e->lead : e->r0;
   e->M0 = M / e->q0;
   e->W = (e->r0 + 31) / 32;
   e->threads = d->threads;  // 0: chosen by occupancy once the tables are sized
@@ -689,7 +725,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   std::vector<uint32_t> e_sorted(r0), inner_ids(r0);
   for (uint32_t i = 0; i < r0; ++i) {
     e_sorted[i] = e_of[order[i]];
-    inner_ids[i] = order[i];
+    inner_ids[i] = order[i] * e->lead;  // idx offset of the slot (ring-0 digit x lead)
   }
   for (uint32_t i = 1; i < r0; ++i)
     if (e_sorted[i] == e_sorted[i - 1]) return fail(ES_ERR_INVALID, "repeated ring-0 residue");
@@ -714,8 +750,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     ++m;
   }
   e->S = (uint32_t)S;
-  e->R_outer = (uint64_t)(R / r0);
-  e->strips = e->R_outer / e->S;
+  e->R_outer = (uint64_t)(R / e->unit);
+  e->strips = (uint64_t)(R / r0) / e->S;
   const uint32_t nb = w - 1 - m;
 
   // ---- mask tables: rows [0, 2m) of 2W words (per ring-0 word: slots without / with an inner
@@ -735,6 +771,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // 2: one row of W words per h (bit i set iff j(h) <= i < r0).  W > 2: j(h) as u16, each
   // carry word built from it (word rows would not fit).
   const uint32_t jbytes = (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
+  e->kind = (jbytes == 1 ? 1u : 0u) | (e->lead > 1 ? 2u : 0u);
+  if (e->lead > 1 && (W != 1 || d->global_masks != 0))
+    return fail(ES_ERR_INVALID, "swapped lead rings need a ring 0 of at most 32 residues and "
+                                "every mask in LDS");
   if (r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
   const uint32_t carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
   std::vector<uint32_t> staged;  // indices into the descriptor's mask list
@@ -861,7 +901,11 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     t_l.push_back(0);
   }
 
-  // ---- strip rows over rings 1..m (ring 1 fastest): H, 0, L (4 dwords), 0, 0, Q8_f
+  // ---- strip rows over rings 1..m (ring 1 fastest): H, X, L (4 dwords), 0, 0, Q8_f.  X is
+  // the idx correction of swapped lead rings (0 otherwise): ring 1 is then the reference's
+  // ring 0 (lead | S), a slot of outer index o = (strip base) + sg has idx r0 * (strip base) +
+  // sg % lead + lead * r0 * (sg / lead) + (slot offset), and X = that - r0 * o - (slot offset)
+  // as a signed 32-bit value (|X| < r0 * S < 2^31).
   const uint32_t F = e->F;
   const uint32_t stride = (8 + F + 7) & ~7u;
   std::vector<uint32_t> strip((size_t)S * stride, 0);
@@ -874,6 +918,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     }
     uint32_t* row = &strip[sg * stride];
     row[0] = (uint32_t)(B / e->M0);
+    row[1] = (uint32_t)(int32_t)((int64_t)(sg % e->lead + (uint64_t)e->lead * r0 * (sg / e->lead)) -
+                                 (int64_t)r0 * (int64_t)sg);
     const u128 L = B % e->M0;
     row[2] = (uint32_t)L;
     row[3] = (uint32_t)(L >> 32);
@@ -940,7 +986,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   ES_HIP(hipGetDevice(&e->device));
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
-  e->kern = pick_kernel((int)e->F, W, G, jbytes == 1 ? 1u : 0u);
+  e->kern = pick_kernel((int)e->F, W, G, e->kind);
   if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
   if (!e->threads) {
     // The workgroup size that keeps the most waves resident per CU (ties: the smaller one).
@@ -1092,10 +1138,13 @@ int es_engine_scan_window(es_engine* e, uint64_t idx_lo, uint64_t idx_hi) {
   if (!e) return fail(ES_ERR_INVALID, "null engine");
   if (idx_hi > (uint64_t)e->R || idx_lo > idx_hi) return fail(ES_ERR_INVALID, "window outside [0, R)");
   if (idx_lo == idx_hi) return ES_OK;
-  const uint64_t o_lo = idx_lo / e->r0, o_hi = (idx_hi + e->r0 - 1) / e->r0;
-  // equal launches of at most launch_outer outer indices (no short tail launch)
+  uint64_t o_lo, o_hi;
+  outer_range(e, idx_lo, idx_hi, &o_lo, &o_hi);
+  // equal launches of at most launch_outer outer indices (no short tail launch), each a
+  // multiple of lead so it covers whole runs of idx
   const uint64_t n = (o_hi - o_lo + e->launch_outer - 1) / e->launch_outer;
-  const uint64_t per = (o_hi - o_lo + n - 1) / n;
+  uint64_t per = (o_hi - o_lo + n - 1) / n;
+  per = (per + e->lead - 1) / e->lead * e->lead;
   for (uint64_t a = o_lo; a < o_hi; a += per) {
     const uint64_t b = std::min(o_hi, a + per);
     const int rc = launch_range(e, a, b, idx_lo, idx_hi);
@@ -1108,7 +1157,7 @@ int es_engine_scan(es_engine* e, uint64_t ostart, uint64_t ocount) {
   if (!e) return fail(ES_ERR_INVALID, "null engine");
   if (ostart > e->R_outer || ocount > e->R_outer - ostart)
     return fail(ES_ERR_INVALID, "outer range outside [0, R_outer)");
-  return es_engine_scan_window(e, ostart * e->r0, (ostart + ocount) * e->r0);
+  return es_engine_scan_window(e, ostart * e->unit, (ostart + ocount) * e->unit);
 }
 
 int es_engine_sync(es_engine* e) {
@@ -1163,7 +1212,9 @@ int es_engine_collect(es_engine* e, uint64_t* ids, uint64_t cap, uint64_t* n, ui
   for (uint64_t v : buf) {
     uint64_t idx = v;
     u128 s = 0;
-    for (uint32_t j = 0; j < e->w; ++j) {
+    for (uint32_t jj = 0; jj < e->w; ++jj) {
+      // idx digits in the reference order: swapped lead rings take ring 1's digit first
+      const uint32_t j = e->lead > 1 && jj < 2 ? 1 - jj : jj;
       const uint64_t dj = idx % e->radix[j];
       idx /= e->radix[j];
       s += e->terms[e->term_off[j] + dj];
@@ -1392,7 +1443,7 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   std::vector<Key> keys;
   for (uint32_t i = 0; i < b->jobs.size(); ++i) {
     const es_engine* e = b->jobs[i].e;
-    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0, e->hp.carry_bytes);
+    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0, e->kind);
     if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
     const uint32_t occ = std::min<uint32_t>(2048u / e->threads,
                                             163840u / std::max<uint32_t>(e->lds_bytes, 1));
@@ -1424,8 +1475,7 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
       J.plan = (uint32_t)(plans.size() - L.plan_off);
       plans.push_back(j.hp);
       L.lds_max = std::max(L.lds_max, e->lds_bytes);
-      J.o_lo = j.idx_lo / e->r0;
-      J.o_hi = (j.idx_hi + e->r0 - 1) / e->r0;
+      outer_range(e, j.idx_lo, j.idx_hi, &J.o_lo, &J.o_hi);
       J.u_lo = J.o_lo / e->S;
       J.u_hi = (J.o_hi + e->S - 1) / e->S;
       J.idx_lo = j.idx_lo;

@@ -84,6 +84,13 @@ def kernel_rings(plan: Plan, max_inner: int = 64):
     return g_mods + mods[1:], g_radix + radix[1:], g_terms + terms[1:], p ** T, f
 
 
+def swap_lead(radix, inner_mod: int = 0) -> bool:
+    """Whether the kernel sweeps the plan's ring 1 as its ring 0 (rings 0 and 1 swapped): when
+    ring 0 has 33..64 residues (two-word mask rows) and ring 1 at most 32 (one word), and ring
+    0 is not split into digit rings.  idx keeps the plan's order; the engine maps it."""
+    return (not inner_mod and len(radix) >= 2 and 32 < radix[0] <= 64 and radix[1] <= 32)
+
+
 @functools.lru_cache(maxsize=512)
 def _table_words(k: int, p: int, T: int) -> np.ndarray:
     """pack_bits of the admissible residues mod p^T (memoised; read-only)."""
@@ -169,6 +176,12 @@ class _Desc:
         M = plan.M
         mods, radix, rterms, self.inner_mod, self.f = kernel_rings(
             plan, int(tun.get("mi_inner_radix", 128)))
+        self.lead = 0
+        if tun.get("mi_ring_swap", "auto") == "auto" and swap_lead(radix, self.inner_mod):
+            self.lead = radix[0]
+            mods = [mods[1], mods[0]] + mods[2:]
+            radix = [radix[1], radix[0]] + radix[2:]
+            rterms = [rterms[1], rterms[0]] + rterms[2:]
         terms = [t for ring in rterms for t in ring]
         self.r0 = radix[0]
         self.q0 = self.inner_mod or mods[0]
@@ -244,6 +257,7 @@ class _Desc:
         d.global_masks = int(tun.get("mi_global_masks", 0))
         d.launch_log2 = int(tun.get("mi_launch_log2", 37))
         d.inner_mod = self.inner_mod
+        d.lead_radix = self.lead
         self.desc = d
 
 

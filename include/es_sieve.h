@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ES_ABI_VERSION 1u
+#define ES_ABI_VERSION 2u
 
 typedef enum es_status {
   ES_OK = 0,
@@ -50,7 +50,14 @@ typedef struct es_engine es_engine;
  * Each group is passed with modulus p^(digits in the group) and its terms (rho * basis) mod M,
  * rho = the group's digit value sum_j d_j p^j.  When ring 0 is such a group, inner_mod is the
  * whole ring's p^T: ring-0 terms are then multiples of M / inner_mod, the split the kernel
- * uses. */
+ * uses.
+ *
+ * The first two rings may be passed swapped instead (lead_radix > 1): ring 0 is then the
+ * reference's ring 1 and ring 1 the reference's ring 0, whose radix lead_radix is.  idx keeps
+ * the reference's order, idx = d + lead_radix * (i + r0 * v) for the reference's ring-0 digit
+ * d, ring-0 slot i of the kernel and the remaining digits v; windows, es_engine_scan units
+ * (lead_radix positions each) and es_engine_collect follow it.  A reference ring 0 of 33..64
+ * residues then gives way to a narrower kernel ring 0 with one-word mask rows. */
 typedef struct es_plan_desc {
   uint32_t abi_version;        /* ES_ABI_VERSION */
   uint32_t k;
@@ -82,6 +89,8 @@ typedef struct es_plan_desc {
   uint32_t global_masks;       /* masks read through L1 instead of LDS: 0, 2 or 4 (r0 <= 32) */
   uint32_t launch_log2;        /* positions per kernel launch, log2 (0 = 37) */
   uint32_t inner_mod;          /* modulus of the ring ring 0 is a digit group of (0 = ring_mod[0]) */
+  uint32_t lead_radix;         /* 0 or 1: rings in the reference order; > 1: rings 0 and 1
+                                  swapped, lead_radix = ring_radix[1] (see above) */
 } es_plan_desc;
 
 typedef struct es_engine_info {

@@ -246,11 +246,12 @@ def test_masks_not_prime_to_M_are_skipped(monkeypatch):
 @pytest.mark.parametrize("k", [301, 399])
 def test_auto_workgroup_size(k):
     """threads="auto" picks the workgroup size of 256, 512, 1024 that keeps the most waves
-    resident per CU: k=301's default wheel stages ~61 KB of two-word mask rows, which at 256
-    threads leaves two workgroups (8 waves) per CU, so a larger size must win there.  Every
-    size scans the oracle's survivors."""
+    resident per CU: k=301's default wheel, kept in the plan's ring order, stages ~61 KB of
+    two-word mask rows, which at 256 threads leaves two workgroups (8 waves) per CU, so a
+    larger size must win there.  Every size scans the oracle's survivors."""
     wheel = K399_PRIMARY if k == 399 else None
-    eng, plan = engine(k, wheel)
+    keep = {"mi_ring_swap": 0} if k == 301 else {}
+    eng, plan = engine(k, wheel, **keep)
     say(f"case k={k} auto threads={eng.threads} lds={eng.info.lds_bytes} r0={eng.r0}")
     assert eng.threads in (256, 512, 1024)
     if k == 301:
@@ -259,7 +260,7 @@ def test_auto_workgroup_size(k):
     span = min(1 << 22, plan.R // 2)
     lo = rng.randrange(0, plan.R - span)
     for th in ("auto", 256, 1024):
-        window_check(k, wheel, 1, lo, lo + span, limit=8, threads=th)
+        window_check(k, wheel, 1, lo, lo + span, limit=8, threads=th, **keep)
 
 
 @pytest.mark.parametrize("k,tune", [(399, {}), (399, {"mi_global_masks": 2}), (300, {})])
@@ -274,3 +275,34 @@ def test_carry_table_forms(k, tune):
     span = min(1 << 22, plan.R // 2)
     lo = rng.randrange(0, plan.R - span)
     window_check(k, wheel, 3, lo, lo + span, limit=8, **tune)
+
+
+@pytest.mark.parametrize("k", [301, 309, 312])
+def test_swapped_lead_rings(k):
+    """A wheel whose ring 0 has 33..64 residues and ring 1 at most 32 runs with the two rings
+    swapped in the kernel (one-word mask rows) while idx keeps the plan's order: unaligned
+    windows (with their minimum), outer-unit launches and the unswapped engine all give the
+    oracle's survivors."""
+    eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22)
+    say(f"case k={k} radix {plan.radix[:3]} kernel r0={eng._d.r0} lead={eng._d.lead}")
+    assert eng._d.lead == plan.radix[0] > 32 and eng._d.r0 == plan.radix[1] <= 32
+    rng = random.Random(k)
+    for span in (1 << 22, 12345, 1):
+        lo = rng.randrange(0, plan.R - span)
+        window_check(k, None, 2, lo, lo + span, limit=8)
+    eng.set_block(4)
+    eng.out_n.fill(0)
+    o0 = plan.R // eng.r0 // 3
+    eng._launch(o0, 70001)
+    eng._launch(o0 + 70001, 3)
+    n = int(eng.out_n.get()[0])
+    a = np.sort(np.asarray([int(x) for x in eng.out[:n].get()], dtype=np.uint64))
+    lo, hi = o0 * eng.r0, (o0 + 70004) * eng.r0
+    nw, want = O.c_scan(k, plan.wheel, plan.rest[:8], 4, lo, hi, threads=16, cap=1 << 22)
+    assert n == nw and np.array_equal(a, want)
+    b, _ = eng.scan_window(4, lo, hi)
+    assert np.array_equal(np.sort(np.asarray(b, dtype=np.uint64)), want)
+    eng2, _ = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, mi_ring_swap=0)
+    assert eng2._d.lead == 0
+    c, _ = eng2.scan_window(4, lo, hi)
+    assert np.array_equal(np.sort(np.asarray(c, dtype=np.uint64)), want)

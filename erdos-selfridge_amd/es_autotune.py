@@ -1,7 +1,8 @@
 """Autotuner of the sieve launch shape for one wheel.
 
-Searches strip-mine length (mi_strip_target), workgroup size (threads) and LDS table layout
-(number of word masks, masks served through L1) by timing a probe window on the GPU (device
+Searches strip-mine length (mi_strip_target), workgroup size (threads), LDS table layout
+(number of word masks, masks served through L1) and the lead-ring swap (mi_ring_swap) by
+timing a probe window on the GPU (device
 time from HIP events), and returns the fastest setting.  The setting changes speed only: every
 candidate setting scans the same positions with the same exact predicate.  Results can be kept
 in a JSON cache keyed by (k, wheel, device).
@@ -9,7 +10,7 @@ in a JSON cache keyed by (k, wheel, device).
 Two search methods: "grid" times every point of the space; "coord" (coordinate descent) starts
 from the default setting and sweeps one dimension at a time with the others held, keeping a
 change only when it beats the current best by more than `margin`, until a full sweep changes
-nothing -- 14 probes per sweep over the default space instead of 120.
+nothing -- 13 probes per sweep over the default space instead of 400.
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ SPACE = {
     "threads": ["auto", 64, 128, 256, 512],
     "mi_masks": ["auto", 4, 6, 8],
     "mi_global_masks": [0, 2],
+    "mi_ring_swap": ["auto", 0],
 }
 
 

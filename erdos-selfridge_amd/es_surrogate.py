@@ -232,14 +232,21 @@ class PlanSpace:
         p0 = order[:, 0]
         r0 = rr_all[:, 0]
         r0k = self.inner[p0, Ei[rows, p0]]
-        W = np.ceil(r0k / 32.0)
         split = r0k < r0
         nr = (rr_all > 0).sum(1)
+        # the engine's lead-ring swap (gpu_sieve_mi355x.swap_lead): ring 0 of 33..64 residues,
+        # ring 1 of <= 32, no split -> the kernel sweeps ring 1 as its ring 0
+        r1 = rr_all[:, 1] if P > 1 else np.zeros(n)
+        swap = ~split & (r0 > 32) & (r0 <= 64) & (r1 > 0) & (r1 <= 32)
+        r0k = np.where(swap, r1, r0k)
+        W = np.ceil(r0k / 32.0)
         # kernel radices: [inner, rest of ring 0 (if split), rings 1..]; 0 = absent
         rest = rr_all[:, 1:]
         kr_split = np.concatenate([r0k[:, None], (r0 / r0k)[:, None], rest], axis=1)
         kr_whole = np.concatenate([r0[:, None], rest, np.zeros((n, 1))], axis=1)
-        kr = np.where(split[:, None], kr_split, kr_whole)
+        kr_swap = np.concatenate([r1[:, None], r0[:, None], rr_all[:, 2:], np.zeros((n, 1))],
+                                 axis=1)
+        kr = np.where(split[:, None], kr_split, np.where(swap[:, None], kr_swap, kr_whole))
         S, m = self._strips(kr)
         nk = (kr > 0).sum(1)
         mr = np.sort(self._take(self.mrate, E), axis=1)[:, :N_LEAD]

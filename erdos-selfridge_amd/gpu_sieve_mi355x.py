@@ -177,7 +177,8 @@ class _Desc:
         mods, radix, rterms, self.inner_mod, self.f = kernel_rings(
             plan, int(tun.get("mi_inner_radix", 128)))
         self.lead = 0
-        if tun.get("mi_ring_swap", "auto") == "auto" and swap_lead(radix, self.inner_mod):
+        if (tun.get("mi_ring_swap", "auto") == "auto" and not int(tun.get("mi_global_masks", 0))
+                and swap_lead(radix, self.inner_mod)):
             self.lead = radix[0]
             mods = [mods[1], mods[0]] + mods[2:]
             radix = [radix[1], radix[0]] + radix[2:]

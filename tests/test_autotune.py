@@ -21,7 +21,8 @@ def _surface(best):
 
 @pytest.mark.parametrize("method", ["coord", "grid"])
 def test_search_finds_peak(monkeypatch, method):
-    best = {"mi_strip_target": 4096, "threads": 128, "mi_masks": 8, "mi_global_masks": 0}
+    best = {"mi_strip_target": 4096, "threads": 128, "mi_masks": 8, "mi_global_masks": 0,
+            "mi_ring_swap": 0}
     monkeypatch.setattr(AT, "probe", _surface(best))
     got, trials = AT.autotune(30, probe_log2=12, method=method)
     assert got == best

@@ -35,7 +35,9 @@ def test_plan_space_matches_plan(k):
         assert abs(F[i, S.N_RING_FEAT] * 8 - math.log2(P.radix[0])) < 1e-5
         assert F[i, 0] * 16 == pytest.approx(math.log2(P.rings[0][0]), abs=1e-5)
         # kernel geometry: inner radix and strip length as the engine derives them
-        _, kr, _, _, _ = G.kernel_rings(P, 128)
+        _, kr, _, inner, _ = G.kernel_rings(P, 128)
+        if G.swap_lead(kr, inner):
+            kr = [kr[1], kr[0]] + kr[2:]
         assert F[i, 16] * 8 == pytest.approx(math.log2(kr[0]), abs=1e-5)
         assert F[i, 19] * 16 == pytest.approx(math.log2(_engine_strip(kr)), abs=1e-5)
 

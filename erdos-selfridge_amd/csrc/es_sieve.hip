@@ -99,8 +99,9 @@ struct HotPlan {
 
 // Exact re-test of the candidate slots in `bits` (sorted ring-0 slots 32*w + b) of outer
 // index o (strip row sg); h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.
-// C & 2: the kernel's rings 0 and 1 are the reference's rings 1 and 0; a slot's idx is then
-// o * r0 + its offset + the strip row's signed correction (dword 1 of the row).  Kernels with
+// CF & 2: the kernel's rings 0 and 1 are the reference's rings 1 and 0, and `o` is passed as
+// the outer index's idx base, o * r0 + the strip row's signed correction (dword 1 of the row):
+// a slot's idx is that + its offset.  Kernels with
 // one or two words per row inline it into the queue drain (an out-of-line call makes the whole
 // kernel allocate registers around the call: 96 VGPRs and spills against 79 and none inline
 // for 6 masks, one word); wider rows, whose unrolled word loop has one drain per word, call
@@ -122,8 +123,7 @@ __device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, ui
   while (bits) {
     const uint32_t i = 32u * w + (uint32_t)__builtin_ctz(bits);
     bits &= bits - 1u;
-    uint64_t idx = o * r0 + C->inner_ids[i];
-    if (CF & 2) idx += (uint64_t)(int64_t)(int32_t)row[1];
+    const uint64_t idx = (CF & 2) ? o + C->inner_ids[i] : o * r0 + C->inner_ids[i];
     if (idx < idx_lo || idx >= idx_hi) continue;
     uint32_t hh = h + C->e0[i];
     if (hh >= q0) hh -= q0;
@@ -212,8 +212,11 @@ __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint3
     const bool kap = es_ge128(Lh, Ll, szh, szl);
     uint32_t h = shu + row[0] + (kap ? 1u : 0u);
     if (h >= P.q0) h -= P.q0;
+    // swapped lead rings: the idx base of the outer index (see es_exact_test)
+    const uint64_t ob = (C & 2) ? (sob + sg) * P.r0 + (uint64_t)(int64_t)(int32_t)row[1]
+                                : sob + sg;
     if (W == 1 || W == 2)
-      es_exact_test<C>(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
+      es_exact_test<C>(P.cold, ent.x, w, h, kap, szl, szh, sg, ob, idx_lo, idx_hi);
     else
       es_exact_test_call(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
   }

@@ -306,3 +306,18 @@ def test_swapped_lead_rings(k):
     assert eng2._d.lead == 0
     c, _ = eng2.scan_window(4, lo, hi)
     assert np.array_equal(np.sort(np.asarray(c, dtype=np.uint64)), want)
+
+
+@pytest.mark.parametrize("k", [100, 309])
+def test_window_edges(k):
+    """Edge windows against the oracle: empty, the block's first and last positions, and a
+    window crossing an outer-index boundary by one position (k=309 runs swapped lead rings,
+    whose windows cover whole runs of lead * r0 positions)."""
+    eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22)
+    ids, mn = eng.scan_window(2, 1000, 1000)
+    assert list(ids) == [] and mn is None
+    r = plan.radix[0]
+    span = min(1 << 20, plan.R // 2)
+    for lo, hi in ((0, span), (plan.R - span, plan.R), (plan.R - 3, plan.R),
+                   (5 * r - 1, 5 * r + 1)):
+        window_check(k, None, 2, lo, hi, limit=8)

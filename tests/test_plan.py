@@ -179,3 +179,38 @@ def test_kernel_digit_rings_keep_idx_order():
     assert G.kernel_rings(P, 128)[1][:2] == [88, 12]
     P399 = EP.Plan(30)
     assert G.kernel_rings(P399, 64)[3:] == (0, 1)
+
+
+@pytest.mark.parametrize("k", [301, 309, 312])
+def test_swapped_lead_rings_keep_idx(k):
+    """Wheels whose ring 0 has 33..64 residues and ring 1 at most 32 are passed with rings 0
+    and 1 swapped: for kernel outer index o = (strip base) + sg and kernel ring-0 digit s, the
+    engine's idx (r0 * o + lead * s + X_sg, X_sg = sg % lead + lead * r0 * (sg // lead) - r0 *
+    sg, with the strip of whole runs of the swapped ring) names the position whose value is
+    the kernel-order CRT sum (the plan's idx_to_n of block 0), and es_engine_scan's outer
+    units are the plan's."""
+    import gpu_sieve_mi355x as G
+    P = EP.Plan(k)
+    d = G._Desc(P, dict(P.tun, mi_inner_radix=128))
+    rad = [int(x) for x in d.ring_radix]
+    lead, r0 = d.lead, rad[0]
+    assert lead == P.radix[0] and rad[:2] == [P.radix[1], P.radix[0]] and r0 <= 32 < lead
+    assert G.swap_lead(P.radix) and not G.swap_lead(rad)
+    offs = [0]
+    for r in rad:
+        offs.append(offs[-1] + r)
+    terms = [int(lo) | (int(hi) << 64) for lo, hi in zip(d.term_lo, d.term_hi)]
+    S = lead * rad[2]                                  # a strip of whole runs (lead | S)
+    rng = random.Random(k)
+    for _ in range(500):
+        o, s = rng.randrange(P.R // r0), rng.randrange(r0)
+        digs, x = [s], o
+        for r in rad[1:]:
+            digs.append(x % r)
+            x //= r
+        n = sum(terms[offs[j] + digs[j]] for j in range(len(rad))) % P.M
+        sg = o % S
+        X = sg % lead + lead * r0 * (sg // lead) - r0 * sg
+        idx = r0 * o + lead * s + X
+        assert idx == o % lead + lead * (s + r0 * (o // lead))
+        assert 0 <= idx < P.R and P.idx_to_n(idx, 0) == n

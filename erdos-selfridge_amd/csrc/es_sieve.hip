@@ -84,9 +84,9 @@ struct HotPlan {
   uint64_t M0_lo, M0_hi;
   uint32_t q0, r0, S, nb, W, stride;
   uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
-  uint32_t carry_bytes, pad;       // 1: the carry table holds j(h) bytes (kernel instance C & 1)
   const uint32_t* lds_src;         // device copy of the LDS image
-  const uint32_t* strip;           // [S][stride] H, 0, L(4), 0, 0, Q8_f (row byte step per f)
+  const uint32_t* strip;           // [S][stride] H, X, L(4), 0, 0, Q8_f (row byte step per f);
+                                   // X = idx correction of swapped lead rings
   const uint32_t* base_radix;      // [nb]
   const uint32_t* base_off;        // [nb] first term of each base ring
   const uint32_t* term_h;          // base-ring terms, h part
@@ -1056,7 +1056,6 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   P.stride = stride;
   P.lds_words = (uint32_t)lds.size();
   P.carry_base = carry_off * 4;
-  P.carry_bytes = jbytes == 1 ? 1u : 0u;
   P.lds_src = (const uint32_t*)(base + o_lds);
   P.strip = (const uint32_t*)(base + o_strip);
   P.base_radix = (const uint32_t*)(base + o_brad);

@@ -51,7 +51,7 @@ TUNABLES = {
     "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
     "mi_inner_radix": "auto",  # ring 0 mod p^T wider than this is swept as digit rings (0:
                                # never; "auto": the engine times the distinct splits)
-    "mi_ring_swap": "auto",    # "auto": a ring 0 of 33..64 residues trades places with a
+    "mi_ring_swap": "auto",    # "auto": a ring 0 of 33..128 residues trades places with a
                                # ring 1 of <= 32 in the kernel (one-word mask rows); 0: never
 }
 

@@ -234,10 +234,10 @@ class PlanSpace:
         r0k = self.inner[p0, Ei[rows, p0]]
         split = r0k < r0
         nr = (rr_all > 0).sum(1)
-        # the engine's lead-ring swap (gpu_sieve_mi355x.swap_lead): ring 0 of 33..64 residues,
+        # the engine's lead-ring swap (gpu_sieve_mi355x.swap_lead): ring 0 of 33..128 residues,
         # ring 1 of <= 32, no split -> the kernel sweeps ring 1 as its ring 0
         r1 = rr_all[:, 1] if P > 1 else np.zeros(n)
-        swap = ~split & (r0 > 32) & (r0 <= 64) & (r1 > 0) & (r1 <= 32)
+        swap = ~split & (r0 > 32) & (r0 <= 128) & (r1 > 0) & (r1 <= 32)
         r0k = np.where(swap, r1, r0k)
         W = np.ceil(r0k / 32.0)
         # kernel radices: [inner, rest of ring 0 (if split), rings 1..]; 0 = absent

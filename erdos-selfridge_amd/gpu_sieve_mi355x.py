@@ -86,9 +86,10 @@ def kernel_rings(plan: Plan, max_inner: int = 64):
 
 def swap_lead(radix, inner_mod: int = 0) -> bool:
     """Whether the kernel sweeps the plan's ring 1 as its ring 0 (rings 0 and 1 swapped): when
-    ring 0 has 33..64 residues (two-word mask rows) and ring 1 at most 32 (one word), and ring
-    0 is not split into digit rings.  idx keeps the plan's order; the engine maps it."""
-    return (not inner_mod and len(radix) >= 2 and 32 < radix[0] <= 64 and radix[1] <= 32)
+    ring 0 has 33..128 residues (two to four words per mask row) and ring 1 at most 32 (one
+    word), and ring 0 is not split into digit rings.  idx keeps the plan's order; the engine
+    maps it."""
+    return (not inner_mod and len(radix) >= 2 and 32 < radix[0] <= 128 and radix[1] <= 32)
 
 
 @functools.lru_cache(maxsize=512)

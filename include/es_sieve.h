@@ -56,8 +56,8 @@ typedef struct es_engine es_engine;
  * reference's ring 1 and ring 1 the reference's ring 0, whose radix lead_radix is.  idx keeps
  * the reference's order, idx = d + lead_radix * (i + r0 * v) for the reference's ring-0 digit
  * d, ring-0 slot i of the kernel and the remaining digits v; windows, es_engine_scan units
- * (lead_radix positions each) and es_engine_collect follow it.  A reference ring 0 of 33..64
- * residues then gives way to a narrower kernel ring 0 with one-word mask rows. */
+ * (lead_radix positions each) and es_engine_collect follow it.  A reference ring 0 of more
+ * than 32 residues then gives way to a narrower kernel ring 0 with one-word mask rows. */
 typedef struct es_plan_desc {
   uint32_t abi_version;        /* ES_ABI_VERSION */
   uint32_t k;

@@ -181,9 +181,9 @@ def test_kernel_digit_rings_keep_idx_order():
     assert G.kernel_rings(P399, 64)[3:] == (0, 1)
 
 
-@pytest.mark.parametrize("k", [301, 309, 312])
+@pytest.mark.parametrize("k", [301, 309, 312, 378])
 def test_swapped_lead_rings_keep_idx(k):
-    """Wheels whose ring 0 has 33..64 residues and ring 1 at most 32 are passed with rings 0
+    """Wheels whose ring 0 has 33..128 residues and ring 1 at most 32 are passed with rings 0
     and 1 swapped: for kernel outer index o = (strip base) + sg and kernel ring-0 digit s, the
     engine's idx (r0 * o + lead * s + X_sg, X_sg = sg % lead + lead * r0 * (sg // lead) - r0 *
     sg, with the strip of whole runs of the swapped ring) names the position whose value is

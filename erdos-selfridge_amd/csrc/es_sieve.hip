@@ -540,8 +540,9 @@ struct es_engine {
   uint32_t n_filters = 0, n_slow = 0, lds_bytes = 0, max_grid = 1;
   u128 M = 1, M0 = 1, R = 1;
   uint64_t R_outer = 1, strips = 1, out_cap = 1u << 20, launch_outer = 1;
-  uint32_t lead = 1;   // > 1: rings 0 and 1 swapped (the reference's ring-0 radix)
-  uint32_t unit = 1;   // positions per es_engine_scan outer unit: lead if swapped, else r0
+  uint32_t lead = 1;   // > 1: the reference's ring j leads (product of the radices before it)
+  uint32_t lead_rings = 0;  // j: the kernel's rings 1..j are the reference's rings 0..j-1
+  uint32_t unit = 1;   // positions per es_engine_scan outer unit: the reference's ring-0 radix
   uint32_t kind = 0;   // kernel instance C: 1 byte carry table, 2 swapped lead rings' idx form
   std::vector<uint32_t> filt_q, slow_order, mask_mod, mask_src, radix;
   std::vector<uint32_t> mask_info, slow_info;  // host copies of the device records
@@ -693,10 +694,15 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     return fail(ES_ERR_INVALID, "inner_mod must divide M, be a multiple of ring_mod[0] and < 2^31");
   e->r0 = d->ring_radix[0];
   e->lead = d->lead_radix > 1 ? d->lead_radix : 1;
-  if (e->lead > 1 && (w < 2 || d->ring_radix[1] != e->lead || d->inner_mod != 0))
-    return fail(ES_ERR_INVALID, "lead_radix needs two rings, ring_radix[1] == lead_radix and no "
-                                "digit split of ring 0");
-  e->unit = e->lead > 1 ? e->lead : e->r0;
+  if (e->lead > 1) {
+    // the lead block: rings 1..lead_rings, whose radices multiply to lead_radix
+    uint64_t L = 1;
+    while (L < e->lead && 1 + e->lead_rings < w) L *= d->ring_radix[1 + e->lead_rings++];
+    if (L != e->lead || d->inner_mod != 0)
+      return fail(ES_ERR_INVALID, "lead_radix must be the product of ring_radix[1..j] for some j "
+                                  "(and ring 0 not a digit split)");
+  }
+  e->unit = e->lead > 1 ? d->ring_radix[1] : e->r0;  // the reference's ring-0 radix
   e->M0 = M / e->q0;
   e->W = (e->r0 + 31) / 32;
   e->threads = d->threads;  // 0: chosen by occupancy once the tables are sized
@@ -742,7 +748,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   while (1 + m < w) {
     const uint64_t nxt = S * d->ring_radix[1 + m];
     if (nxt > (1u << 22) || nxt * e->W >= (1u << kQueueIndexBits)) break;
-    if (m > 0 && nxt > target) {
+    if (m > 0 && m >= e->lead_rings && nxt > target) {  // the lead block is always whole
       if (nxt * S < (uint64_t)target * target) {
         S = nxt;
         ++m;
@@ -752,6 +758,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     S = nxt;
     ++m;
   }
+  if (m < e->lead_rings) return fail(ES_ERR_LIMIT, "lead rings exceed the strip limits");
   e->S = (uint32_t)S;
   e->R_outer = (uint64_t)(R / e->unit);
   e->strips = (uint64_t)(R / r0) / e->S;
@@ -1215,8 +1222,10 @@ int es_engine_collect(es_engine* e, uint64_t* ids, uint64_t cap, uint64_t* n, ui
     uint64_t idx = v;
     u128 s = 0;
     for (uint32_t jj = 0; jj < e->w; ++jj) {
-      // idx digits in the reference order: swapped lead rings take ring 1's digit first
-      const uint32_t j = e->lead > 1 && jj < 2 ? 1 - jj : jj;
+      // idx digits in the reference order: with a lead block the kernel's rings 1..j come
+      // first, then its ring 0
+      const uint32_t lj = e->lead_rings;
+      const uint32_t j = lj && jj <= lj ? (jj < lj ? jj + 1 : 0) : jj;
       const uint64_t dj = idx % e->radix[j];
       idx /= e->radix[j];
       s += e->terms[e->term_off[j] + dj];

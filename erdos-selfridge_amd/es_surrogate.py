@@ -30,7 +30,7 @@ import time
 import numpy as np
 
 import es_plan as EP
-from gpu_sieve_mi355x import AUTO_MASK_STOP
+from gpu_sieve_mi355x import AUTO_MASK_STOP, LEAD_CAP, LEAD_GAIN
 
 N_FEAT = 64
 N_RING_FEAT = 8      # rings described by (log2 q, log2 radix)
@@ -234,20 +234,29 @@ class PlanSpace:
         r0k = self.inner[p0, Ei[rows, p0]]
         split = r0k < r0
         nr = (rr_all > 0).sum(1)
-        # the engine's lead-ring swap (gpu_sieve_mi355x.swap_lead): ring 0 of 33..128 residues,
-        # ring 1 of <= 32, no split -> the kernel sweeps ring 1 as its ring 0
-        r1 = rr_all[:, 1] if P > 1 else np.zeros(n)
-        swap = ~split & (r0 > 32) & (r0 <= 128) & (r1 > 0) & (r1 <= 32)
-        r0k = np.where(swap, r1, r0k)
+        # the engine's lead block (gpu_sieve_mi355x.swap_lead): ring 0 of 33..128 residues, not
+        # split, ring j the first of <= 32, rings 0..j-1 holding <= LEAD_CAP positions, ring j
+        # filling its word LEAD_GAIN times better (or ring 0 > 96) -> the kernel sweeps ring j
+        # as its ring 0 and rings 0..j-1 as its first strip rings
+        narrow = (rr_all[:, 1:] > 0) & (rr_all[:, 1:] <= 32)
+        jl = np.argmax(narrow, axis=1) + 1
+        cum = np.cumprod(np.where(rr_all > 0, rr_all, 1.0), axis=1)
+        lead = cum[rows, jl - 1]
+        rj = rr_all[rows, jl]
+        fill0 = r0 / (32.0 * np.ceil(r0 / 32.0))
+        swap = (~split & (r0 > 32) & (r0 <= 128) & narrow.any(1) & (lead <= LEAD_CAP)
+                & ((r0 > 96) | (rj / 32.0 >= LEAD_GAIN * fill0)))
+        r0k = np.where(swap, rj, r0k)
         W = np.ceil(r0k / 32.0)
         # kernel radices: [inner, rest of ring 0 (if split), rings 1..]; 0 = absent
         rest = rr_all[:, 1:]
         kr_split = np.concatenate([r0k[:, None], (r0 / r0k)[:, None], rest], axis=1)
         kr_whole = np.concatenate([r0[:, None], rest, np.zeros((n, 1))], axis=1)
-        kr_swap = np.concatenate([r1[:, None], r0[:, None], rr_all[:, 2:], np.zeros((n, 1))],
-                                 axis=1)
+        pos = np.arange(P)[None, :]
+        perm = np.where(pos == 0, jl[:, None], np.where(pos <= jl[:, None], pos - 1, pos))
+        kr_swap = np.concatenate([np.take_along_axis(rr_all, perm, 1), np.zeros((n, 1))], axis=1)
         kr = np.where(split[:, None], kr_split, np.where(swap[:, None], kr_swap, kr_whole))
-        S, m = self._strips(kr)
+        S, m = self._strips(kr, mmin=np.where(swap, jl, 0))
         nk = (kr > 0).sum(1)
         mr = np.sort(self._take(self.mrate, E), axis=1)[:, :N_LEAD]
         tail = np.log2(mr).sum(1) + np.log2(r0k)
@@ -300,11 +309,12 @@ class PlanSpace:
         return F
 
     @staticmethod
-    def _strips(kr, target: float = 1024.0, cap: float = float(1 << 22)):
+    def _strips(kr, target: float = 1024.0, cap: float = float(1 << 22), mmin=None):
         """Strip length S and strip ring count m for kernel radices kr[:, 0] (ring 0), kr[:, 1:]
-        (0 = absent): the leading product nearest the target in log scale, as the engine
-        chooses it."""
+        (0 = absent): the leading product nearest the target in log scale, with at least
+        mmin strip rings (a lead block is always whole), as the engine chooses it."""
         n = kr.shape[0]
+        mmin = np.zeros(n, dtype=np.int64) if mmin is None else np.asarray(mmin)
         S = np.ones(n)
         m = np.zeros(n, dtype=np.int64)
         done = np.zeros(n, dtype=bool)
@@ -315,7 +325,7 @@ class PlanSpace:
             over = live & (nxt > cap)
             done |= over | ~live
             live &= ~over
-            cross = live & (m > 0) & (nxt > target)
+            cross = live & (m > 0) & (m >= mmin) & (nxt > target)
             take_cross = cross & (nxt * S < target * target)
             step = live & (~cross | take_cross)
             S = np.where(step, nxt, S)

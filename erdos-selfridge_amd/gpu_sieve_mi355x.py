@@ -84,12 +84,36 @@ def kernel_rings(plan: Plan, max_inner: int = 64):
     return g_mods + mods[1:], g_radix + radix[1:], g_terms + terms[1:], p ** T, f
 
 
-def swap_lead(radix, inner_mod: int = 0) -> bool:
-    """Whether the kernel sweeps the plan's ring 1 as its ring 0 (rings 0 and 1 swapped): when
-    ring 0 has 33..128 residues (two to four words per mask row) and ring 1 at most 32 (one
-    word), and ring 0 is not split into digit rings.  idx keeps the plan's order; the engine
-    maps it."""
-    return (not inner_mod and len(radix) >= 2 and 32 < radix[0] <= 128 and radix[1] <= 32)
+LEAD_CAP = 8192   # positions of the lead block (the rings before the one brought forward)
+LEAD_GAIN = 1.15  # least gain in mask-word use (slots per word bit) that brings a ring forward
+
+
+def swap_lead(radix, inner_mod: int = 0) -> int:
+    """j >= 1 when the kernel sweeps the plan's ring j as its ring 0 (one word per mask row)
+    and the rings 0..j-1 before it as its first strip rings (the lead block): when ring 0 has
+    33..128 residues (two to four words per row) and is not split into digit rings, ring j is
+    the first ring of at most 32, the lead block holds at most LEAD_CAP positions, and ring j
+    fills its word at least LEAD_GAIN times as well as ring 0 fills its words (or ring 0 needs
+    four words).  0 otherwise.  idx keeps the plan's order; the engine maps it.  (Measured on
+    2^36 windows: k=302/308/309/312/386 +35..45 %, k=378/391 1.7x/3.5x; k=360/380/392/400, whose
+    ring 0 fills its words as well as ring j would, lose 6..12 % swapped, so they keep it.)"""
+    if inner_mod or len(radix) < 2 or not 32 < radix[0] <= 128:
+        return 0
+    L = 1
+    for j in range(1, len(radix)):
+        L *= radix[j - 1]
+        if L > LEAD_CAP:
+            return 0
+        if radix[j] <= 32:
+            fill0 = radix[0] / (32 * -(-radix[0] // 32))
+            return j if radix[0] > 96 or radix[j] / 32 >= LEAD_GAIN * fill0 else 0
+    return 0
+
+
+def lead_order(n: int, j: int) -> list[int]:
+    """Kernel ring order for a lead block of j rings: ring j, then rings 0..j-1, then the
+    rest (identity for j = 0)."""
+    return list(range(n)) if not j else [j] + list(range(j)) + list(range(j + 1, n))
 
 
 @functools.lru_cache(maxsize=512)
@@ -178,12 +202,14 @@ class _Desc:
         mods, radix, rterms, self.inner_mod, self.f = kernel_rings(
             plan, int(tun.get("mi_inner_radix", 128)))
         self.lead = 0
-        if (tun.get("mi_ring_swap", "auto") == "auto" and not int(tun.get("mi_global_masks", 0))
-                and swap_lead(radix, self.inner_mod)):
-            self.lead = radix[0]
-            mods = [mods[1], mods[0]] + mods[2:]
-            radix = [radix[1], radix[0]] + radix[2:]
-            rterms = [rterms[1], rterms[0]] + rterms[2:]
+        j = (swap_lead(radix, self.inner_mod) if tun.get("mi_ring_swap", "auto") == "auto"
+             and not int(tun.get("mi_global_masks", 0)) else 0)
+        if j:
+            self.lead = math.prod(radix[:j])
+            order = lead_order(len(radix), j)
+            mods = [mods[i] for i in order]
+            radix = [radix[i] for i in order]
+            rterms = [rterms[i] for i in order]
         terms = [t for ring in rterms for t in ring]
         self.r0 = radix[0]
         self.q0 = self.inner_mod or mods[0]

@@ -277,17 +277,20 @@ def test_carry_table_forms(k, tune):
     window_check(k, wheel, 3, lo, lo + span, limit=8, **tune)
 
 
-@pytest.mark.parametrize("k", [301, 309, 312, 378])
+@pytest.mark.parametrize("k", [301, 309, 312, 378, 363, 386])
 def test_swapped_lead_rings(k):
-    """A wheel whose ring 0 has 33..128 residues and ring 1 at most 32 runs with the two rings
-    swapped in the kernel (one-word mask rows) while idx keeps the plan's order: unaligned
-    windows (with their minimum), outer-unit launches and the unswapped engine all give the
-    oracle's survivors.  (k=378's ring 0 is 3^6 with 108 residues: the automatic inner radix
-    also times its digit splits, so the bound is fixed at 128 here to keep it whole.)"""
+    """A wheel whose ring 0 has 33..128 residues runs with its first ring j of at most 32
+    residues as the kernel's ring 0 and rings 0..j-1 as a lead block behind it (j = 1, or 2 at
+    k=363 and 386), one-word mask rows, while idx keeps the plan's order: unaligned windows
+    (with their minimum), outer-unit launches and the unswapped engine all give the oracle's
+    survivors.  (k=378's ring 0 is 3^6 with 108 residues: the automatic inner radix also times
+    its digit splits, so the bound is fixed at 128 here to keep it whole.)"""
+    import gpu_sieve_mi355x as G
     keep = {"mi_inner_radix": 128} if k == 378 else {}
     eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, **keep)
     say(f"case k={k} radix {plan.radix[:3]} kernel r0={eng._d.r0} lead={eng._d.lead}")
-    assert eng._d.lead == plan.radix[0] > 32 and eng._d.r0 == plan.radix[1] <= 32
+    j = G.swap_lead(plan.radix)
+    assert j and eng._d.lead == np.prod(plan.radix[:j]) and eng._d.r0 == plan.radix[j] <= 32
     rng = random.Random(k)
     for span in (1 << 22, 12345, 1):
         lo = rng.randrange(0, plan.R - span)

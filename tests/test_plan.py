@@ -181,26 +181,27 @@ def test_kernel_digit_rings_keep_idx_order():
     assert G.kernel_rings(P399, 64)[3:] == (0, 1)
 
 
-@pytest.mark.parametrize("k", [301, 309, 312, 378])
-def test_swapped_lead_rings_keep_idx(k):
-    """Wheels whose ring 0 has 33..128 residues and ring 1 at most 32 are passed with rings 0
-    and 1 swapped: for kernel outer index o = (strip base) + sg and kernel ring-0 digit s, the
-    engine's idx (r0 * o + lead * s + X_sg, X_sg = sg % lead + lead * r0 * (sg // lead) - r0 *
-    sg, with the strip of whole runs of the swapped ring) names the position whose value is
-    the kernel-order CRT sum (the plan's idx_to_n of block 0), and es_engine_scan's outer
-    units are the plan's."""
+@pytest.mark.parametrize("k,j", [(301, 1), (309, 1), (312, 1), (378, 1), (363, 2), (386, 2)])
+def test_swapped_lead_rings_keep_idx(k, j):
+    """Wheels whose ring 0 has 33..128 residues are passed with their first ring j of at most
+    32 residues in front and rings 0..j-1 (the lead block, `lead` positions) behind it: for
+    kernel outer index o = (strip base) + sg and kernel ring-0 digit s, the engine's idx
+    (r0 * o + lead * s + X_sg, X_sg = sg % lead + lead * r0 * (sg // lead) - r0 * sg, with a
+    strip of whole lead blocks) names the position whose value is the kernel-order CRT sum
+    (the plan's idx_to_n of block 0)."""
     import gpu_sieve_mi355x as G
     P = EP.Plan(k)
     d = G._Desc(P, dict(P.tun, mi_inner_radix=128))
     rad = [int(x) for x in d.ring_radix]
     lead, r0 = d.lead, rad[0]
-    assert lead == P.radix[0] and rad[:2] == [P.radix[1], P.radix[0]] and r0 <= 32 < lead
-    assert G.swap_lead(P.radix) and not G.swap_lead(rad)
+    assert G.swap_lead(P.radix) == j and lead == math.prod(P.radix[:j])
+    assert rad == [P.radix[i] for i in G.lead_order(len(rad), j)] and r0 <= 32 < P.radix[0]
+    assert not G.swap_lead(rad)
     offs = [0]
     for r in rad:
         offs.append(offs[-1] + r)
     terms = [int(lo) | (int(hi) << 64) for lo, hi in zip(d.term_lo, d.term_hi)]
-    S = lead * rad[2]                                  # a strip of whole runs (lead | S)
+    S = lead * rad[j + 1]                              # a strip of whole lead blocks
     rng = random.Random(k)
     for _ in range(500):
         o, s = rng.randrange(P.R // r0), rng.randrange(r0)
@@ -214,3 +215,16 @@ def test_swapped_lead_rings_keep_idx(k):
         idx = r0 * o + lead * s + X
         assert idx == o % lead + lead * (s + r0 * (o // lead))
         assert 0 <= idx < P.R and P.idx_to_n(idx, 0) == n
+
+
+def test_lead_block_rule():
+    """The lead block is taken only where ring j fills its mask word clearly better than ring 0
+    fills its words (or ring 0 needs four words), within the size cap."""
+    import gpu_sieve_mi355x as G
+    assert G.swap_lead([42, 30, 25]) == 1            # 42/64 of two words -> 30/32 of one
+    assert G.swap_lead([36, 35, 26, 25]) == 2        # lead block 36 x 35
+    assert G.swap_lead([64, 45, 32, 26]) == 0        # 64/64 already full
+    assert G.swap_lead([77, 36, 24, 18]) == 0        # 77/96 vs 24/32
+    assert G.swap_lead([108, 26, 20]) == 1           # four words
+    assert G.swap_lead([40, 30], inner_mod=3 ** 6) == 0 and G.swap_lead([30, 20]) == 0
+    assert G.swap_lead([100, 90, 90, 20]) == 0       # lead block above the cap

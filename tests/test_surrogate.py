@@ -36,21 +36,21 @@ def test_plan_space_matches_plan(k):
         assert F[i, 0] * 16 == pytest.approx(math.log2(P.rings[0][0]), abs=1e-5)
         # kernel geometry: inner radix and strip length as the engine derives them
         _, kr, _, inner, _ = G.kernel_rings(P, 128)
-        if G.swap_lead(kr, inner):
-            kr = [kr[1], kr[0]] + kr[2:]
+        j = G.swap_lead(kr, inner)
+        kr = [kr[i] for i in G.lead_order(len(kr), j)]
         assert F[i, 16] * 8 == pytest.approx(math.log2(kr[0]), abs=1e-5)
-        assert F[i, 19] * 16 == pytest.approx(math.log2(_engine_strip(kr)), abs=1e-5)
+        assert F[i, 19] * 16 == pytest.approx(math.log2(_engine_strip(kr, mmin=j)), abs=1e-5)
 
 
-def _engine_strip(radix, target=1024):
+def _engine_strip(radix, target=1024, mmin=0):
     """The engine's strip rule (es_engine_create): leading radices after ring 0, nearest the
-    target in log scale."""
+    target in log scale, with at least mmin rings (a lead block is always whole)."""
     S, m = 1, 0
     while 1 + m < len(radix):
         nxt = S * radix[1 + m]
         if nxt > (1 << 22):
             break
-        if m > 0 and nxt > target:
+        if m > 0 and m >= mmin and nxt > target:
             if nxt * S < target * target:
                 S = nxt
             break

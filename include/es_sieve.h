@@ -52,11 +52,12 @@ typedef struct es_engine es_engine;
  * whole ring's p^T: ring-0 terms are then multiples of M / inner_mod, the split the kernel
  * uses.
  *
- * The first two rings may be passed swapped instead (lead_radix > 1): ring 0 is then the
- * reference's ring 1 and ring 1 the reference's ring 0, whose radix lead_radix is.  idx keeps
- * the reference's order, idx = d + lead_radix * (i + r0 * v) for the reference's ring-0 digit
- * d, ring-0 slot i of the kernel and the remaining digits v; windows, es_engine_scan units
- * (lead_radix positions each) and es_engine_collect follow it.  A reference ring 0 of more
+ * The reference's ring j >= 1 may be passed first instead (lead_radix > 1): the rings are then
+ * in the order j, 0, 1, .., j-1, j+1, .. and lead_radix is the product of the reference's
+ * radices 0..j-1 (ring_radix[1..j] here, the lead block).  idx keeps the reference's order,
+ * idx = d + lead_radix * (i + r0 * v) for the lead block's digits d (mixed radix), ring-0
+ * slot i of the kernel and the remaining digits v; windows, es_engine_scan units (the
+ * reference's ring-0 radix, ring_radix[1], positions each) and es_engine_collect follow it.  A reference ring 0 of more
  * than 32 residues then gives way to a narrower kernel ring 0 with one-word mask rows. */
 typedef struct es_plan_desc {
   uint32_t abi_version;        /* ES_ABI_VERSION */
@@ -89,8 +90,8 @@ typedef struct es_plan_desc {
   uint32_t global_masks;       /* masks read through L1 instead of LDS: 0, 2 or 4 (r0 <= 32) */
   uint32_t launch_log2;        /* positions per kernel launch, log2 (0 = 37) */
   uint32_t inner_mod;          /* modulus of the ring ring 0 is a digit group of (0 = ring_mod[0]) */
-  uint32_t lead_radix;         /* 0 or 1: rings in the reference order; > 1: rings 0 and 1
-                                  swapped, lead_radix = ring_radix[1] (see above) */
+  uint32_t lead_radix;         /* 0 or 1: rings in the reference order; > 1: the product of
+                                  ring_radix[1..j] of a lead block (see above) */
 } es_plan_desc;
 
 typedef struct es_engine_info {

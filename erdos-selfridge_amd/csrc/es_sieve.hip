@@ -86,7 +86,7 @@ struct HotPlan {
   uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
   const uint32_t* lds_src;         // device copy of the LDS image
   const uint32_t* strip;           // [S][stride] H, X, L(4), 0, 0, Q8_f (row byte step per f);
-                                   // X = idx correction of swapped lead rings
+                                   // X = idx correction of a lead block
   const uint32_t* base_radix;      // [nb]
   const uint32_t* base_off;        // [nb] first term of each base ring
   const uint32_t* term_h;          // base-ring terms, h part
@@ -99,13 +99,13 @@ struct HotPlan {
 
 // Exact re-test of the candidate slots in `bits` (sorted ring-0 slots 32*w + b) of outer
 // index o (strip row sg); h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.
-// CF & 2: the kernel's rings 0 and 1 are the reference's rings 1 and 0, and `o` is passed as
-// the outer index's idx base, o * r0 + the strip row's signed correction (dword 1 of the row):
-// a slot's idx is that + its offset.  Kernels with
-// one or two words per row inline it into the queue drain (an out-of-line call makes the whole
-// kernel allocate registers around the call: 96 VGPRs and spills against 79 and none inline
-// for 6 masks, one word); wider rows, whose unrolled word loop has one drain per word, call
-// it out of line to keep the code small.
+// CF & 2: a lead block (the kernel's ring 0 is the reference's ring j, its rings 1..j the
+// reference's rings 0..j-1), and `o` is passed as the outer index's idx base, o * r0 + the
+// strip row's signed correction (dword 1 of the row): a slot's idx is that + its offset.
+// Kernels with one or two words per row inline it into the queue drain (an out-of-line call
+// makes the whole kernel allocate registers around the call: 96 VGPRs and spills against 79
+// and none inline for 6 masks, one word); wider rows, whose unrolled word loop has one drain
+// per word, call it out of line to keep the code small.
 template <int CF>
 __device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
                                            uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
@@ -212,7 +212,7 @@ __device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint3
     const bool kap = es_ge128(Lh, Ll, szh, szl);
     uint32_t h = shu + row[0] + (kap ? 1u : 0u);
     if (h >= P.q0) h -= P.q0;
-    // swapped lead rings: the idx base of the outer index (see es_exact_test)
+    // lead block: the idx base of the outer index (see es_exact_test)
     const uint64_t ob = (C & 2) ? (sob + sg) * P.r0 + (uint64_t)(int64_t)(int32_t)row[1]
                                 : sob + sg;
     if (W == 1 || W == 2)
@@ -458,7 +458,7 @@ typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
 
 // Instances: every F <= 12 with all masks in LDS for W = 1, 2, 3, 4 and runtime W; for W = 1
 // also G = 2 or 4 of the masks in global memory, and (G = 0) the byte carry table (C & 1)
-// and the swapped lead rings' idx form (C & 2).
+// and the lead block's idx form (C & 2).
 template <int F>
 KernelFn kernel_for(uint32_t W, uint32_t G, uint32_t C) {
   if (C) {
@@ -543,7 +543,7 @@ struct es_engine {
   uint32_t lead = 1;   // > 1: the reference's ring j leads (product of the radices before it)
   uint32_t lead_rings = 0;  // j: the kernel's rings 1..j are the reference's rings 0..j-1
   uint32_t unit = 1;   // positions per es_engine_scan outer unit: the reference's ring-0 radix
-  uint32_t kind = 0;   // kernel instance C: 1 byte carry table, 2 swapped lead rings' idx form
+  uint32_t kind = 0;   // kernel instance C: 1 byte carry table, 2 lead block idx form
   std::vector<uint32_t> filt_q, slow_order, mask_mod, mask_src, radix;
   std::vector<uint32_t> mask_info, slow_info;  // host copies of the device records
   std::vector<u128> terms;                      // all ring terms, ring-major (host idx -> s)
@@ -579,7 +579,7 @@ int upload_tconst(es_engine* e, const std::vector<uint32_t>& tc) {
 }
 
 // Kernel outer indices [o_lo, o_hi) whose positions include every idx in [idx_lo, idx_hi):
-// idx / r0 in the reference order; with swapped lead rings, the whole runs of lead * r0 idx
+// idx / r0 in the reference order; with a lead block, the whole runs of lead * r0 idx
 // (lead outer indices each) that meet the window.  The exact test drops positions outside it.
 void outer_range(const es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t* o_lo,
                  uint64_t* o_hi) {
@@ -783,7 +783,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   const uint32_t jbytes = (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
   e->kind = (jbytes == 1 ? 1u : 0u) | (e->lead > 1 ? 2u : 0u);
   if (e->lead > 1 && (W != 1 || d->global_masks != 0))
-    return fail(ES_ERR_INVALID, "swapped lead rings need a ring 0 of at most 32 residues and "
+    return fail(ES_ERR_INVALID, "a lead block needs a ring 0 of at most 32 residues and "
                                 "every mask in LDS");
   if (r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
   const uint32_t carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
@@ -912,10 +912,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   }
 
   // ---- strip rows over rings 1..m (ring 1 fastest): H, X, L (4 dwords), 0, 0, Q8_f.  X is
-  // the idx correction of swapped lead rings (0 otherwise): ring 1 is then the reference's
-  // ring 0 (lead | S), a slot of outer index o = (strip base) + sg has idx r0 * (strip base) +
-  // sg % lead + lead * r0 * (sg / lead) + (slot offset), and X = that - r0 * o - (slot offset)
-  // as a signed 32-bit value (|X| < r0 * S < 2^31).
+  // the idx correction of a lead block (0 otherwise): rings 1..j are then the reference's
+  // rings 0..j-1 (lead = their radix product divides S), a slot of outer index o = (strip
+  // base) + sg has idx r0 * (strip base) + sg % lead + lead * r0 * (sg / lead) + (slot
+  // offset), and X = that - r0 * o - (slot offset) as a signed 32-bit value (|X| < r0 * S).
   const uint32_t F = e->F;
   const uint32_t stride = (8 + F + 7) & ~7u;
   std::vector<uint32_t> strip((size_t)S * stride, 0);

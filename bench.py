@@ -46,12 +46,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="batch, k200: autotune the launch shape of each wheel first")
-    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "stream", "valu", "batch",
-                                           "k200", "surrogate", "mitm"),
+    ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "stream", "valu", "lds",
+                                           "batch", "k200", "surrogate", "mitm"),
                     default="sieve",
                     help="sieve: k=399 wheel windows (headline); fpass: k=397 candidate filter "
                          "pass; copy: 3 device copies of 1 GiB (calibrates HBM byte counters); "
-                         "valu: INT32 VALU peak microbenchmark; stream: what library "
+                         "valu: INT32 VALU peak microbenchmark; lds: LDS ds_read_b64 "
+                         "peak (conflict-free and random rows); stream: what library "
                          "streaming kernels reach on this device's HBM; "
                          "batch: k=300..320 batched launches; k200: full-power wheel window; "
                          "surrogate: learned planner vs plain planner, k=350..400; "
@@ -80,6 +81,18 @@ def main() -> int:
         return bench_mitm(a)
     if a.workload == "valu":
         return bench_valu(a)
+    if a.workload == "lds":
+        lds = lds_read_peak()
+        print(json.dumps({
+            "metric": "LDS ds_read_b64 bytes/s (conflict-free rows)",
+            "value": lds["conflict_free_bytes_per_s"], "unit": "bytes/s", "n_gpus": 1,
+            "steps": 3, "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"{lds['cus']} CUs x 2 workgroups x 1024 threads, 8192 "
+                                   "rounds of 8 ds_read_b64 per lane, 32 KiB table"},
+            "nominal_peak": LDS_PEAK_NOMINAL, "frac_of_nominal":
+                lds["conflict_free_bytes_per_s"] / LDS_PEAK_NOMINAL, **lds}), flush=True)
+        return 0
     if a.workload == "stream":
         return bench_stream(a)
     if a.workload == "copy":
@@ -131,51 +144,11 @@ def main() -> int:
     assert found[-1][1] == G399, f"last window must return g(399), got {found[-1]}"
     assert all(c == 0 for c, _ in found[:-1]) and found[-1][0] == 1
 
-    ef = plan.expected_filter_evals()
-    ops_ref = 12 + 10 * ef                              # reference-algorithm ops per position
     kern_rate = dev_pos / (dev_ms / 1e3)
     F = eng.info.n_masks_used
-    # Profiled figures of es_sieve_kernel<6, 0, 1, 1> on this wheel (rocprofv3, 2^36-position
-    # launches): 33.6 VALU instructions per wave per outer index (SQ_INSTS_VALU; one outer
-    # index = r0 positions per lane), LDS busy 89 % of the kernel's cycles (SQ_LDS_IDX_ACTIVE,
-    # 54 % bank-conflict stalls), HBM traffic 1.47 MB per launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
-    # copy-calibrated).  Other instances are not profiled.
-    profiled = F == 6 and eng.info.words == 1
-    ops_kernel = 33.6 / eng.r0 if profiled else None
-    achieved = ops_ref * kern_rate
-    roofline = {"bound": "valu-int32", "achieved": achieved / 1e12, "peak": INT32_PEAK / 1e12,
-                "unit": "TOP/s", "frac": achieved / INT32_PEAK,
-                "traffic": 1.47e6 if profiled else None,
-                "ops_per_position_reference": round(ops_ref, 3),
-                "kernel_ops_per_position": round(ops_kernel, 3) if ops_kernel else None,
-                "kernel_frac": ops_kernel * kern_rate / INT32_PEAK if ops_kernel else None,
-                "lds_busy_frac_profiled": 0.89 if profiled else None,
-                "limiter_profiled": ("LDS: random mask-row reads, 54 % of cycles bank-conflict "
-                                     "stalls" if profiled else None),
-                "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
-                "kernel_positions_per_s": kern_rate}
-
-    cpu = None
-    if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import es_oracle as O
-        import numpy as np
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        lo = g_idx - 50_000
-        probe = 1 << 20
-        c0 = time.perf_counter()
-        O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + probe, threads=threads)
-        per = (time.perf_counter() - c0) / probe
-        span = max(probe, min(int(a.cpu_seconds / per), 1 << 36))
-        c0 = time.perf_counter()
-        n_cpu, ids_cpu = O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + span, threads=threads)
-        cdt = time.perf_counter() - c0
-        gpu_ids, _ = eng.scan_window(BLOCK, lo, lo + span)
-        assert np.array_equal(np.sort(np.asarray(gpu_ids, dtype=np.uint64)), ids_cpu)
-        cpu = {"value": span / cdt, "unit": "positions/s", "cores": threads, "kind": "port",
-               "sample": f"positions [{lo}, {lo + span}) of block {BLOCK} (holds g(399)); "
-                         f"C restatement of idx_to_n + Kummer digit test per filter prime; "
-                         f"survivors identical to the GPU's ({n_cpu})"}
+    lds_peak = lds_read_peak()
+    roofline = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
+    cpu = None if a.no_cpu else cpu_baseline(plan, eng, g_idx, a)
 
     print(json.dumps({
         "metric": "residue positions sieved/s (k=399 wheel); time to first nonempty block vs plain",
@@ -189,6 +162,119 @@ def main() -> int:
         "roofline": roofline, "cpu_baseline": cpu,
     }), flush=True)
     return 0
+
+
+# LDS read peak of gfx950 at the nominal clock: 256 CUs x 256 bytes per CU-clock (64 banks of
+# 4 bytes) x 2.4 GHz.  `--workload lds` measures the conflict-free ds_read_b64 rate live.
+LDS_PEAK_NOMINAL = 256 * 256 * 2.4e9
+# Profiled figures of es_sieve_kernel<6, 0, 1, 1> on the k=399 primary wheel (rocprofv3, 2^36-
+# position launches; the counters are per wave-step = one outer index of 64 lanes = 1856
+# positions).  Other instances are not profiled.
+K399_PROFILE = {"sq_lds_idx_active_per_wave_step": 39.7, "sq_lds_bank_conflict_per_wave_step": 24.0,
+                "sq_insts_valu_per_wave_step": 33.6, "hbm_bytes_per_launch": 1.47e6}
+
+
+def lds_read_peak(blocks_per_cu: int = 2, iters: int = 8192) -> dict:
+    """Conflict-free and random-row ds_read_b64 rates of this device (es_calib_lds), best of 3."""
+    import ctypes as C
+
+    import es_native as nat
+    cus = nat.device_query()["cus"] or 256     # no torch: the engine's HIP runtime is loaded
+    out = {"cus": cus}
+    for mode, name in ((0, "conflict_free"), (1, "random_rows")):
+        best = 0.0
+        for _ in range(3):
+            ms, nb = C.c_double(0), C.c_uint64(0)
+            nat.check(nat.lib().es_calib_lds(cus * blocks_per_cu, iters, mode, C.byref(ms),
+                                             C.byref(nb)))
+            best = max(best, nb.value / (ms.value * 1e-3))
+        out[name + "_bytes_per_s"] = best
+    out["bytes_per_cu_clock_at_2p4GHz"] = out["conflict_free_bytes_per_s"] / (cus * 2.4e9)
+    return out
+
+
+def sieve_roofline(eng, plan, kern_rate: float, dev_ms: float, launches: int, lds: dict) -> dict:
+    """Roofline of the sieve kernel against the LDS read peak: the word-mask kernel reads no HBM
+    per position; its per-position work is LDS mask-row reads (lds_bytes_per_position) plus a few
+    32-bit VALU ops.  `achieved` = algorithmic LDS bytes per position x positions per second of
+    kernel time; `peak` = the conflict-free ds_read_b64 rate measured in this run.  The
+    reference algorithm's INT32 op count (12 + 10 E_f per position) is reported apart: this
+    kernel does not execute that algorithm, so its rate is no fraction of a hardware peak."""
+    bpp = eng.lds_bytes_per_position()
+    peak = lds["conflict_free_bytes_per_s"]
+    achieved = bpp * kern_rate
+    profiled = eng.info.n_masks_used == 6 and eng.info.words == 1 and eng.r0 == 29
+    ef = plan.expected_filter_evals()
+    ops_ref = 12 + 10 * ef
+    return {
+        "bound": "lds", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "GB/s",
+        "frac": achieved / peak,
+        "traffic": K399_PROFILE["hbm_bytes_per_launch"] if profiled else None,
+        "lds_bytes_per_position": bpp,
+        "lds_peak_nominal_GBps": LDS_PEAK_NOMINAL / 1e9,
+        "lds_random_rows_GBps": lds["random_rows_bytes_per_s"] / 1e9,
+        "lds_bytes_per_cu_clock": lds["bytes_per_cu_clock_at_2p4GHz"],
+        "lds_cycles_per_wave_step_profiled":
+            K399_PROFILE["sq_lds_idx_active_per_wave_step"] if profiled else None,
+        "lds_conflict_cycles_per_wave_step_profiled":
+            K399_PROFILE["sq_lds_bank_conflict_per_wave_step"] if profiled else None,
+        "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
+        "kernel_positions_per_s": kern_rate,
+        "reference_algorithm": {
+            "int32_ops_per_position": round(ops_ref, 3), "E_f": round(ef, 4),
+            "equivalent_int32_ops_per_s": ops_ref * kern_rate,
+            "int32_valu_peak": INT32_PEAK,
+            "note": "ops the reference's per-position loop would execute; this kernel runs "
+                    "the word-mask algorithm instead, so this is work avoided, not a roofline"},
+    }
+
+
+def host_cpu() -> dict:
+    """Host cores this process may use, nproc, and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    use = min(aff, cap) if cap > 0 else aff
+    return {"threads": max(1, use), "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": cap or None, "cpu_model": model}
+
+
+def cpu_baseline(plan, eng, g_idx: int, a) -> dict:
+    """The oracle's C restatement of the reference CPU path (idx_to_n + Kummer digit test per
+    filter prime, tests/test_enumeration.py:37-43) on every host core this process may use (the
+    GPU's CPU share where OMP_NUM_THREADS caps it), on a bounded sub-window of block 6 holding
+    g(399); the GPU must return the same survivors there."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import es_oracle as O
+    import numpy as np
+    hc = host_cpu()
+    threads = a.cpu_threads or hc["threads"]
+    lo = g_idx - 50_000
+    probe = 1 << 20
+    c0 = time.perf_counter()
+    O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + probe, threads=threads)
+    per = (time.perf_counter() - c0) / probe
+    span = max(probe, min(int(a.cpu_seconds / per), 1 << 36))
+    c0 = time.perf_counter()
+    n_cpu, ids_cpu = O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + span, threads=threads)
+    cdt = time.perf_counter() - c0
+    gpu_ids, _ = eng.scan_window(BLOCK, lo, lo + span)
+    assert np.array_equal(np.sort(np.asarray(gpu_ids, dtype=np.uint64)), ids_cpu)
+    return {"value": span / cdt, "unit": "positions/s", "cores": threads, "kind": "port",
+            "nproc": hc["nproc"], "affinity": hc["affinity"], "cpu_model": hc["cpu_model"],
+            "sample": f"positions [{lo}, {lo + span}) of block {BLOCK} (holds g(399)), "
+                      f"{cdt:.1f} s on {threads} threads; C restatement of idx_to_n + Kummer "
+                      f"digit test per filter prime; survivors identical to the GPU's ({n_cpu})"}
 
 
 HBM_PEAK = 8.0e12      # bytes/s
@@ -384,11 +470,24 @@ def bench_batch(a) -> int:
         "config": {"workload": "21 plans x 4 blocks x 2^34 positions", "jobs": len(jobs),
                    "launches": launches, "autotuned": bool(a.autotune),
                    "tuned": {str(k): v for k, v in tuned.items()}},
+        "roofline": batch_roofline(engines, jobs, N, pos / (bms / 1e3), bms, launches),
         "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
         "block_sweep": {"jobs": len(sw_jobs), "blocks": len(blocks), "window": 1 << 28,
                         **sweep},
         "survivors_total": sum(len(x) for x in single)}), flush=True)
     return 0
+
+
+def batch_roofline(engines, jobs, N: int, rate: float, ms: float, launches: int) -> dict:
+    """LDS roofline of a batch: algorithmic LDS bytes per position averaged over the jobs (each
+    job's wheel has its own mask count, words per row and ring-0 radix), times the batch's
+    positions per second of device time, against the measured conflict-free LDS read rate."""
+    lds = lds_read_peak()
+    bpp = sum(engines[k].lds_bytes_per_position() for k, _, _, _ in jobs) / len(jobs)
+    peak = lds["conflict_free_bytes_per_s"]
+    return {"bound": "lds", "achieved": bpp * rate / 1e9, "peak": peak / 1e9, "unit": "GB/s",
+            "frac": bpp * rate / peak, "traffic": None, "lds_bytes_per_position": bpp,
+            "kernel_ms": ms, "launches": launches}
 
 
 def bench_k200(a) -> int:
@@ -439,7 +538,6 @@ def bench_k200(a) -> int:
         cpu = {"value": sub / cdt, "unit": "positions/s", "cores": threads, "kind": "port",
                "sample": f"first 2^28 positions of the window ({cdt:.1f} s), survivors agree"}
     kern_rate = dev_pos / (dev_ms / 1e3)
-    ops_ref = 12 + 10 * plan.expected_filter_evals()
     print(json.dumps({
         "metric": "residue positions sieved/s (k=200 full-power wheel)", "value": a.steps * N / wall,
         "unit": "positions/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -454,12 +552,7 @@ def bench_k200(a) -> int:
                    "threads": eng.threads},
         "survivors": len(ns), "survivor_hash": O.survivor_hash(ns),
         "min": str(mn) if mn is not None else None,
-        "roofline": {"bound": "valu-int32", "achieved": ops_ref * kern_rate / 1e12,
-                     "peak": INT32_PEAK / 1e12, "unit": "TOP/s",
-                     "frac": ops_ref * kern_rate / INT32_PEAK, "traffic": None,
-                     "ops_per_position_reference": round(ops_ref, 3),
-                     "kernel_avg_launch_ms": dev_ms / max(1, launches),
-                     "kernel_positions_per_s": kern_rate},
+        "roofline": sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_read_peak()),
         "cpu_baseline": cpu}), flush=True)
     return 0
 

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "es_common.h"
+#include "es_sieve_dev.h"
 
 namespace es {
 
@@ -57,470 +58,26 @@ using es::mk128;
 using es::mod_u128;
 using es::inv_mod;
 
+using namespace es_dev;
+
 namespace {
 
-constexpr int kMaxMasks = 12;
-constexpr int kMaskInfo = 8;   // dwords per mask record
-constexpr int kSlowInfo = 16;  // dwords per exact-filter record
-
-// Read only by the exact re-test of queued candidates, through one pointer.
-struct ColdPlan {
-  uint64_t M0_lo, M0_hi;
-  uint32_t q0, r0, n_slow, stride;
-  uint32_t slow_fp, pad;      // 1: every exact-filter modulus < 2^18 (FP64 residues are exact)
-  const uint32_t* strip;      // strip rows; L_sigma at dwords 2..5
-  const uint32_t* e0;         // [r0] ring-0 e values, ascending
-  const uint32_t* inner_ids;  // [r0] idx offset of each sorted slot: its ring-0 digit x lead
-  const uint32_t* slow;       // [n_slow][16] q, word offset, 2^32, 2^64, 2^96 mod q, t*M mod q,
-                              // 2 unused, then 2^32, 2^64, 2^96 mod q and 1/q as doubles
-  const uint32_t* slow_bits;  // admissibility bitmaps
-  unsigned long long* out;    // survivor IDs
-  unsigned long long* out_n;  // survivor counter (may pass out_cap)
-  uint64_t out_cap;
-};
-
-// Kernel argument.
-struct HotPlan {
-  uint64_t M0_lo, M0_hi;
-  uint32_t q0, r0, S, nb, W, stride;
-  uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
-  const uint32_t* lds_src;         // device copy of the LDS image
-  const uint32_t* strip;           // [S][stride] H, X, L(4), 0, 0, Q8_f (row byte step per f);
-                                   // X = idx correction of a lead block
-  const uint32_t* base_radix;      // [nb]
-  const uint32_t* base_off;        // [nb] first term of each base ring
-  const uint32_t* term_h;          // base-ring terms, h part
-  const uint64_t* term_l;          // base-ring terms, l part (lo, hi pairs)
-  const uint32_t* masks;           // [F][8] m, table byte base, M0 mod m, 2^32, 2^64, 2^96 mod m,
-                                   //        row multiplier (-M)^-1 mod m, t*M mod m
-  const uint32_t* gmask;           // mask tables served from global memory (through L1)
-  const ColdPlan* cold;
-};
-
-// Exact re-test of the candidate slots in `bits` (sorted ring-0 slots 32*w + b) of outer
-// index o (strip row sg); h, kap are the outer index's ring-0 row and l-carry, z = M0 - l_u.
-// CF & 2: a lead block (the kernel's ring 0 is the reference's ring j, its rings 1..j the
-// reference's rings 0..j-1), and `o` is passed as the outer index's idx base, o * r0 + the
-// strip row's signed correction (dword 1 of the row): a slot's idx is that + its offset.
-// Kernels with one or two words per row inline it into the queue drain (an out-of-line call
-// makes the whole kernel allocate registers around the call: 96 VGPRs and spills against 79
-// and none inline for 6 masks, one word); wider rows, whose unrolled word loop has one drain
-// per word, call it out of line to keep the code small.
-template <int CF>
-__device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
-                                           uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
-                                           uint64_t zh, uint32_t sg, uint64_t o,
-                                           uint64_t idx_lo, uint64_t idx_hi) {
-  const uint32_t* row = C->strip + (size_t)sg * C->stride;
-  const u128 L = es::mk128((uint64_t)row[2] | ((uint64_t)row[3] << 32),
-                       (uint64_t)row[4] | ((uint64_t)row[5] << 32));
-  const u128 M0 = es::mk128(C->M0_lo, C->M0_hi);
-  const u128 z = es::mk128(zl, zh);
-  u128 l = (M0 - z) + L;  // l_u + L_sigma
-  if (kap) l -= M0;
-  const uint32_t q0 = C->q0, r0 = C->r0, ns = C->n_slow;
-  const bool fp = C->slow_fp != 0;
-  while (bits) {
-    const uint32_t i = 32u * w + (uint32_t)__builtin_ctz(bits);
-    bits &= bits - 1u;
-    const uint64_t idx = (CF & 2) ? o + C->inner_ids[i] : o * r0 + C->inner_ids[i];
-    if (idx < idx_lo || idx >= idx_hi) continue;
-    uint32_t hh = h + C->e0[i];
-    if (hh >= q0) hh -= q0;
-    const u128 s = (u128)hh * M0 + l;
-    const uint64_t s_lo = (uint64_t)s, s_hi = (uint64_t)(s >> 64);
-    const double d0 = (double)(uint32_t)s_lo, d1 = (double)(uint32_t)(s_lo >> 32),
-                 d2 = (double)(uint32_t)s_hi, d3 = (double)(uint32_t)(s_hi >> 32);
-    bool ok = true;
-    for (uint32_t j = 0; j < ns && ok; ++j) {
-      cu32_t* f = (cu32_t*)C->slow + kSlowInfo * j;  // same j in every active lane: scalar
-      const uint32_t q = f[0];
-      uint32_t r;
-      if (fp) {
-        // every term below 2^50 and the sum below 2^53: exact; the quotient from one multiply
-        // by 1/q is off by at most one either way
-        cf64_t* fd = reinterpret_cast<cf64_t*>(f + 8);
-        const double qd = (double)q;
-        const double acc = fma(d3, fd[2], fma(d2, fd[1], fma(d1, fd[0], d0 + (double)f[5])));
-        double x = fma(-floor(acc * fd[3]), qd, acc);
-        x = x < 0.0 ? x + qd : x;
-        x = x >= qd ? x - qd : x;
-        r = (uint32_t)x;
-      } else {
-        r = es_mod128(s_lo, s_hi, q, f[2], f[3], f[4]) + f[5];
-        if (r >= q) r -= q;
-      }
-      ok = (C->slow_bits[f[1] + (r >> 5)] >> (r & 31u)) & 1u;
-    }
-    if (ok) {
-      const unsigned long long slot = atomicAdd(C->out_n, 1ull);
-      if (slot < C->out_cap) C->out[slot] = idx;
-    }
-  }
-}
-
-__device__ __noinline__ void es_exact_test_call(const ColdPlan* __restrict__ C, uint32_t bits,
-                                                uint32_t w, uint32_t h, uint32_t kap,
-                                                uint64_t zl, uint64_t zh, uint32_t sg,
-                                                uint64_t o, uint64_t idx_lo, uint64_t idx_hi) {
-  es_exact_test<0>(C, bits, w, h, kap, zl, zh, sg, o, idx_lo, idx_hi);
-}
-
-// Per-wave candidate queue in LDS, after the tables (kWaveQueueBytes per wave): up to
-// kQueueCap entries {bits, (sg * words + w) | lane << 26} of mask words that still have
-// candidates.  Entries wait until 64 are queued and are then re-tested exactly, one per lane,
-// so the exact test runs on full waves; a strip's leftovers are re-tested when it ends.  The
-// entry's lane supplies its strip data (z, outer base, h_u) through cross-lane permutes.
-constexpr uint32_t kQueueCap = 128;
-constexpr uint32_t kWaveQueueBytes = kQueueCap * 8;
-constexpr uint32_t kQueueIndexBits = 26;  // sg * words + w < 2^26 (checked at engine creation)
-
-__device__ __forceinline__ uint2* es_wave_queue(const HotPlan& P, const uint32_t* lds) {
-  const uint32_t base = (P.lds_words * 4u + 15u) & ~15u;
-  return reinterpret_cast<uint2*>(reinterpret_cast<char*>(const_cast<uint32_t*>(lds)) + base +
-                                  (threadIdx.x >> 6) * kWaveQueueBytes);
-}
-
-__device__ __forceinline__ uint32_t es_from_lane(uint32_t src, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
-}
-
-// Exact re-test of queued entries [first, first + n), one per lane.  Called by the whole wave
-// (the permutes read every lane's strip data).
-template <int W, int C>
-__device__ __forceinline__ void es_drain(const HotPlan& P, const uint2* q, uint32_t first,
-                                         uint32_t n, uint32_t hu, uint64_t zl, uint64_t zh,
-                                         uint64_t obase, uint64_t idx_lo, uint64_t idx_hi) {
-  __builtin_amdgcn_wave_barrier();
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint2 ent = lane < n ? q[first + lane] : make_uint2(0u, 0u);
-  const uint32_t src = ent.y >> kQueueIndexBits;
-  const uint64_t szl = (uint64_t)es_from_lane(src, (uint32_t)zl) |
-                       ((uint64_t)es_from_lane(src, (uint32_t)(zl >> 32)) << 32);
-  const uint64_t szh = (uint64_t)es_from_lane(src, (uint32_t)zh) |
-                       ((uint64_t)es_from_lane(src, (uint32_t)(zh >> 32)) << 32);
-  const uint64_t sob = (uint64_t)es_from_lane(src, (uint32_t)obase) |
-                       ((uint64_t)es_from_lane(src, (uint32_t)(obase >> 32)) << 32);
-  const uint32_t shu = es_from_lane(src, hu);
-  if (lane < n) {
-    const uint32_t sw = ent.y & ((1u << kQueueIndexBits) - 1u);
-    const uint32_t nw = W ? (uint32_t)W : P.W;
-    const uint32_t sg = W == 1 ? sw : sw / nw, w = sw - sg * nw;
-    const uint32_t* row = P.strip + (size_t)sg * P.stride;
-    const uint64_t Ll = (uint64_t)row[2] | ((uint64_t)row[3] << 32);
-    const uint64_t Lh = (uint64_t)row[4] | ((uint64_t)row[5] << 32);
-    const bool kap = es_ge128(Lh, Ll, szh, szl);
-    uint32_t h = shu + row[0] + (kap ? 1u : 0u);
-    if (h >= P.q0) h -= P.q0;
-    // lead block: the idx base of the outer index (see es_exact_test)
-    const uint64_t ob = (C & 2) ? (sob + sg) * P.r0 + (uint64_t)(int64_t)(int32_t)row[1]
-                                : sob + sg;
-    if (W == 1 || W == 2)
-      es_exact_test<C>(P.cold, ent.x, w, h, kap, szl, szh, sg, ob, idx_lo, idx_hi);
-    else
-      es_exact_test_call(P.cold, ent.x, w, h, kap, szl, szh, sg, sob + sg, idx_lo, idx_hi);
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Queues this lane's candidate word (bits != 0); drains 64 entries once that many wait.
-// `cnt` is wave-uniform.
-template <int W, int C>
-__device__ __forceinline__ void es_enqueue(const HotPlan& P, uint2* q, uint32_t& cnt,
-                                           uint32_t bits, uint32_t sw, uint32_t hu, uint64_t zl,
-                                           uint64_t zh, uint64_t obase, uint64_t idx_lo,
-                                           uint64_t idx_hi) {
-  const uint64_t any = __ballot(bits != 0u);
-  if (any == 0) return;
-  if (bits) {
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
-    q[cnt + below] = make_uint2(bits, sw | ((threadIdx.x & 63u) << kQueueIndexBits));
-  }
-  cnt += (uint32_t)__popcll(any);
-  if (cnt >= 64u) {
-    cnt -= 64u;
-    es_drain<W, C>(P, q, cnt, 64u, hu, zl, zh, obase, idx_lo, idx_hi);
-  }
-}
-
-// One strip: S outer indices, uniform across the wave.  W = words per ring-0 mask (0 = runtime).
-// The last G of the F masks are read from global memory (vector loads through L1) instead of
-// LDS, so the two load pipes share the row traffic.  A[f] is the byte address of the lane's
-// row in table f; an outer sum that wraps past M starts one row on.  The words for slots
-// without an inner wrap and with one are ANDed over the masks first and merged by the carry
-// word once.
-template <int F, int G, int W, int C, bool CHECK>
-__device__ __forceinline__ void es_sweep(const HotPlan& P, const uint32_t* lds,
-                                         const uint32_t (&A)[F + 1], uint32_t hu, uint64_t zl,
-                                         uint64_t zh, uint64_t obase, uint32_t sb, uint32_t se,
-                                         uint2* wq, uint64_t idx_lo, uint64_t idx_hi) {
-  const uint32_t q0 = P.q0, S = P.S, stride = P.stride;
-  uint32_t cnt = 0;
-  const uint32_t nw = W ? (uint32_t)W : P.W;
-  const char* lb = reinterpret_cast<const char*>(lds);
-  const char* gb = reinterpret_cast<const char*>(P.gmask);
-  const uint32_t* carry_tab = reinterpret_cast<const uint32_t*>(lb + P.carry_base);
-  for (uint32_t sg = 0; sg < S; ++sg) {
-    cu32_t* row = (cu32_t*)(P.strip) + (size_t)sg * stride;
-    const uint32_t Hs = row[0];
-    const uint64_t Ll = (uint64_t)row[2] | ((uint64_t)row[3] << 32);
-    const uint64_t Lh = (uint64_t)row[4] | ((uint64_t)row[5] << 32);
-    const bool kap = es_ge128(Lh, Ll, zh, zl);      // l_u + L_sigma >= M0
-    uint32_t h = hu + Hs + (kap ? 1u : 0u);
-    const bool c1 = h >= q0;                        // b_u + B_sigma >= M
-    h = c1 ? h - q0 : h;
-    const bool live = !CHECK || (sg >= sb && sg < se);
-    const uint32_t wrap = c1 ? 8u * nw : 0u;        // one row on
-    if (W == 1) {
-      uint32_t carry;
-      if (C & 1) {  // byte table of j(h): bits i >= j carry
-        const uint32_t j = reinterpret_cast<const uint8_t*>(carry_tab)[h];
-        carry = j >= 32u ? 0u : 0xffffffffu << j;
-      } else {
-        carry = carry_tab[h];
-      }
-      uint32_t bx = 0xffffffffu, by = 0xffffffffu;
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        const uint32_t a = A[f] + row[8 + f] + wrap;
-        const uint2 v = f < F - G ? *reinterpret_cast<const uint2*>(lb + a)
-                                  : *reinterpret_cast<const uint2*>(gb + a);
-        bx &= v.x;
-        by &= v.y;
-      }
-      uint32_t bits = (carry & by) | (~carry & bx);
-      if (F == 0) bits = P.r0 >= 32 ? 0xffffffffu : ((1u << P.r0) - 1u);
-      if (!live) bits = 0;
-      es_enqueue<W, C>(P, wq, cnt, bits, sg, hu, zl, zh, obase, idx_lo, idx_hi);
-    } else if (W == 2) {
-      const uint2 carry = *reinterpret_cast<const uint2*>(carry_tab + 2 * h);
-      uint32_t b0x = 0xffffffffu, b0y = 0xffffffffu, b1x = 0xffffffffu, b1y = 0xffffffffu;
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        const uint32_t a = A[f] + row[8 + f] + wrap;
-        const uint4 v = *reinterpret_cast<const uint4*>(lb + a);
-        b0x &= v.x;
-        b0y &= v.y;
-        b1x &= v.z;
-        b1y &= v.w;
-      }
-      uint32_t b0 = (carry.x & b0y) | (~carry.x & b0x);
-      uint32_t b1 = (carry.y & b1y) | (~carry.y & b1x);
-      if (F == 0) b1 = P.r0 >= 64 ? 0xffffffffu : ((1u << (P.r0 - 32)) - 1u);
-      if (!live) b0 = b1 = 0;
-      es_enqueue<W, C>(P, wq, cnt, b0, 2 * sg, hu, zl, zh, obase, idx_lo, idx_hi);
-      es_enqueue<W, C>(P, wq, cnt, b1, 2 * sg + 1, hu, zl, zh, obase, idx_lo, idx_hi);
-    } else {
-      // carrying slots are the suffix [j, r0) of the sorted ring-0 slots
-      const uint32_t j = reinterpret_cast<const uint16_t*>(carry_tab)[h];
-#pragma unroll
-      for (uint32_t w = 0; w < nw; ++w) {
-        const int d = (int)j - 32 * (int)w;
-        const uint32_t carry = d <= 0 ? 0xffffffffu : (d >= 32 ? 0u : 0xffffffffu << d);
-        uint32_t bx = 0xffffffffu, by = 0xffffffffu;
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          const uint32_t a = A[f] + row[8 + f] + wrap + 8u * w;
-          const uint2 v = *reinterpret_cast<const uint2*>(lb + a);
-          bx &= v.x;
-          by &= v.y;
-        }
-        uint32_t bits = (carry & by) | (~carry & bx);
-        if (F == 0) {
-          const uint32_t n = P.r0 - 32u * w;
-          bits = n >= 32u ? 0xffffffffu : ((1u << n) - 1u);
-        }
-        if (!live) bits = 0;
-        es_enqueue<W, C>(P, wq, cnt, bits, sg * nw + w, hu, zl, zh, obase, idx_lo, idx_hi);
-      }
-    }
-  }
-  if (cnt) es_drain<W, C>(P, wq, 0, cnt, hu, zl, zh, obase, idx_lo, idx_hi);
-}
-
-// Strip u of a plan (u may be a replay of a valid strip for lanes past the range: `in` = false
-// makes its range empty).  Decodes the strip base over rings m+1.. in (h, l) form, sets up the
-// lane's mask-row addresses, then sweeps the strip.
-template <int F, int G, int W, int C>
-__device__ __forceinline__ void es_strip(const HotPlan& P, const uint32_t* lds, uint64_t u,
-                                         bool in, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo,
-                                         uint64_t idx_hi) {
-  const uint64_t M0l = P.M0_lo, M0h = P.M0_hi;
-  const uint32_t q0 = P.q0, S = P.S;
-  const uint32_t row_bytes = 8u * (W ? (uint32_t)W : P.W);
-  cu32_t* mk = (cu32_t*)(P.masks);
-  uint32_t hu = 0;
-  uint64_t ll = 0, lh = 0;
-  uint64_t rem = u;
-  for (uint32_t j = 0; j < P.nb; ++j) {
-    const uint32_t r = P.base_radix[j];
-    uint32_t d;
-    if (j + 1 == P.nb) {
-      d = (uint32_t)rem;
-    } else {
-      const uint64_t qt = rem / r;
-      d = (uint32_t)(rem - qt * r);
-      rem = qt;
-    }
-    const uint32_t o = P.base_off[j] + d;
-    const uint64_t tl = P.term_l[2 * o], th = P.term_l[2 * o + 1];
-    const uint64_t nl = ll + tl;
-    lh = lh + th + (nl < ll ? 1ull : 0ull);
-    ll = nl;
-    uint32_t nh = hu + P.term_h[o];
-    if (es_ge128(lh, ll, M0h, M0l)) {
-      const uint64_t bl = ll < M0l ? 1ull : 0ull;
-      ll -= M0l;
-      lh = lh - M0h - bl;
-      nh += 1u;
-    }
-    if (nh >= q0) nh -= q0;
-    hu = nh;
-  }
-  const uint64_t zl = M0l - ll;  // z = M0 - l_u: l_u + L >= M0 <=> L >= z
-  const uint64_t zh = M0h - lh - (M0l < ll ? 1ull : 0ull);
-  // byte address of the row of this lane's strip-base phase x in each mask table
-  uint32_t A[F + 1];
-#pragma unroll
-  for (int f = 0; f < F; ++f) {
-    cu32_t* rec = mk + kMaskInfo * f;
-    const uint32_t m = rec[0];
-    const uint64_t bm = (uint64_t)hu * rec[2] + es_mod128(ll, lh, m, rec[3], rec[4], rec[5]) +
-                        rec[7];
-    const uint32_t x = (uint32_t)(bm % m);
-    A[f] = rec[1] + row_bytes * (uint32_t)((uint64_t)x * rec[6] % m);
-  }
-  A[F] = 0;
-  const uint64_t obase = u * S;
-  uint2* wq = es_wave_queue(P, lds);
-  uint32_t sb = 0, se = 0;
-  if (in) {
-    sb = o_lo > obase ? (uint32_t)(o_lo - obase) : 0u;
-    se = o_hi - obase >= S ? S : (uint32_t)(o_hi - obase);
-  }
-  const bool full = in && sb == 0 && se == S;
-  if (__all(full))
-    es_sweep<F, G, W, C, false>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
-  else
-    es_sweep<F, G, W, C, true>(P, lds, A, hu, zl, zh, obase, sb, se, wq, idx_lo, idx_hi);
-}
-
-template <int F, int G, int W, int C>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
-es_sieve_kernel(const HotPlan P, uint64_t u_lo, uint64_t u_hi, uint64_t o_lo, uint64_t o_hi,
-                uint64_t idx_lo, uint64_t idx_hi) {
-  extern __shared__ uint32_t lds[];
-  for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
-  __syncthreads();
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  // Wave-uniform strip loop: a lane past u_hi replays strip u_lo with an empty range.
-  for (uint64_t base = u_lo + (uint64_t)blockIdx.x * blockDim.x; base < u_hi; base += step) {
-    const uint64_t my = base + threadIdx.x;
-    const bool in = my < u_hi;
-    es_strip<F, G, W, C>(P, lds, in ? my : u_lo, in, o_lo, o_hi, idx_lo, idx_hi);
-  }
-}
-
-// Grid-mapped batch: one launch over all jobs of a shape; workgroup b serves strips
-// [first_u + (b - wg_start) * blockDim.x, ...) of the job whose workgroup range holds b (found by
-// binary search over the jobs' first workgroups).  The hardware dispatcher hands workgroups to
-// CUs as they free up, so uneven jobs balance at workgroup granularity, as in the single-plan
-// kernel.
-struct GridJob {
-  uint64_t wg_start;           // first workgroup of the job in the launch
-  uint64_t u_lo, u_hi, o_lo, o_hi, idx_lo, idx_hi;
-  uint32_t plan, pad;
-};
-
-template <int F, int G, int W, int C>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
-es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
-                     uint32_t n_jobs) {
-  extern __shared__ uint32_t lds[];
-  const uint64_t b = blockIdx.x;
-  uint32_t lo = 0, hi = n_jobs;  // last job with wg_start <= b
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (jobs[mid].wg_start <= b) lo = mid; else hi = mid;
-  }
-  const GridJob J = jobs[lo];
-  const HotPlan P = plans[J.plan];
-  for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
-  __syncthreads();
-  const uint64_t my = J.u_lo + (b - J.wg_start) * blockDim.x + threadIdx.x;
-  const bool in = my < J.u_hi;
-  es_strip<F, G, W, C>(P, lds, in ? my : J.u_lo, in, J.o_lo, J.o_hi, J.idx_lo, J.idx_hi);
-}
-
-typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
-typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
-
-// Instances: every F <= 12 with all masks in LDS for W = 1, 2, 3, 4 and runtime W; for W = 1
-// also G = 2 or 4 of the masks in global memory, and (G = 0) the byte carry table (C & 1)
-// and the lead block's idx form (C & 2).
-template <int F>
-KernelFn kernel_for(uint32_t W, uint32_t G, uint32_t C) {
-  if (C) {
-    if (W != 1 || G != 0) return nullptr;
-    if (C == 1) return es_sieve_kernel<F, 0, 1, 1>;
-    if (C == 2) return es_sieve_kernel<F, 0, 1, 2>;
-    return C == 3 ? es_sieve_kernel<F, 0, 1, 3> : nullptr;
-  }
-  if (W == 1) {
-    if (G == 2 && F >= 2) return es_sieve_kernel<F, (F >= 2 ? 2 : 0), 1, 0>;
-    if (G == 4 && F >= 4) return es_sieve_kernel<F, (F >= 4 ? 4 : 0), 1, 0>;
-    return G == 0 ? es_sieve_kernel<F, 0, 1, 0> : nullptr;
-  }
-  if (G != 0) return nullptr;
-  if (W == 2) return es_sieve_kernel<F, 0, 2, 0>;
-  if (W == 3) return es_sieve_kernel<F, 0, 3, 0>;
-  if (W == 4) return es_sieve_kernel<F, 0, 4, 0>;
-  return es_sieve_kernel<F, 0, 0, 0>;
-}
-
-template <int F>
-GridBatchFn grid_batch_for(uint32_t W, uint32_t G, uint32_t C) {
-  if (G != 0) return nullptr;
-  if (C) {
-    if (W != 1) return nullptr;
-    if (C == 1) return es_grid_batch_kernel<F, 0, 1, 1>;
-    if (C == 2) return es_grid_batch_kernel<F, 0, 1, 2>;
-    return C == 3 ? es_grid_batch_kernel<F, 0, 1, 3> : nullptr;
-  }
-  if (W == 1) return es_grid_batch_kernel<F, 0, 1, 0>;
-  if (W == 2) return es_grid_batch_kernel<F, 0, 2, 0>;
-  if (W == 3) return es_grid_batch_kernel<F, 0, 3, 0>;
-  if (W == 4) return es_grid_batch_kernel<F, 0, 4, 0>;
-  return es_grid_batch_kernel<F, 0, 0, 0>;
-}
-
-
-#define ES_F_SWITCH(FN, F, ...)        \
-  switch (F) {                         \
-    case 0: return FN<0>(__VA_ARGS__);   \
-    case 1: return FN<1>(__VA_ARGS__);   \
-    case 2: return FN<2>(__VA_ARGS__);   \
-    case 3: return FN<3>(__VA_ARGS__);   \
-    case 4: return FN<4>(__VA_ARGS__);   \
-    case 5: return FN<5>(__VA_ARGS__);   \
-    case 6: return FN<6>(__VA_ARGS__);   \
-    case 7: return FN<7>(__VA_ARGS__);   \
-    case 8: return FN<8>(__VA_ARGS__);   \
-    case 9: return FN<9>(__VA_ARGS__);   \
-    case 10: return FN<10>(__VA_ARGS__); \
-    case 11: return FN<11>(__VA_ARGS__); \
-    case 12: return FN<12>(__VA_ARGS__); \
-    default: return nullptr;           \
-  }
+using namespace es_dev;
 
 KernelFn pick_kernel(int F, uint32_t W, uint32_t G, uint32_t C) {
-  ES_F_SWITCH(kernel_for, F, W, G, C)
+  if (F < 0 || F > kMaxMasks) return nullptr;
+  if (F < kPartFirstF[1]) return pick_kernel_p0(F, W, G, C);
+  if (F < kPartFirstF[2]) return pick_kernel_p1(F, W, G, C);
+  if (F < kPartFirstF[3]) return pick_kernel_p2(F, W, G, C);
+  return pick_kernel_p3(F, W, G, C);
 }
 GridBatchFn pick_grid_batch(int F, uint32_t W, uint32_t G, uint32_t C) {
-  ES_F_SWITCH(grid_batch_for, F, W, G, C)
+  if (F < 0 || F > kMaxMasks) return nullptr;
+  if (F < kPartFirstF[1]) return pick_grid_batch_p0(F, W, G, C);
+  if (F < kPartFirstF[2]) return pick_grid_batch_p1(F, W, G, C);
+  if (F < kPartFirstF[3]) return pick_grid_batch_p2(F, W, G, C);
+  return pick_grid_batch_p3(F, W, G, C);
 }
-
 struct Blob {
   std::vector<uint8_t> bytes;
   size_t put(const void* p, size_t n) {
@@ -549,11 +106,12 @@ struct es_engine {
   std::vector<u128> terms;                      // all ring terms, ring-major (host idx -> s)
   std::vector<uint32_t> term_off;
   uint64_t t = 0;
+  bool t_known = true;       // false after es_engine_set_tconst: the minimum needs the block
   uint64_t last_count = 0;   // survivor count read since the last launch or reset
   bool count_valid = false;
   void* dblob = nullptr;
-  unsigned long long* d_out = nullptr;
-  unsigned long long* d_count = nullptr;
+  unsigned long long* d_out = nullptr;    // [out_cap] IDs, then [out_cap][2] residues
+  unsigned long long* d_count = nullptr;  // count, 1 + slot of the minimum, its floor
   uint32_t* d_mask_info = nullptr;
   uint32_t* d_slow_info = nullptr;
   HotPlan hp{};
@@ -562,18 +120,23 @@ struct es_engine {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   uint64_t launches = 0, positions = 0;
+  uint64_t floor = 0;        // host copy of d_count[2]
 };
 
 namespace {
 
+// Block constants, and the least residue s that counts for the minimum: n = t*M + s > k + 1,
+// i.e. s >= k + 2 in block 0 and every s in later blocks (0 as well while the block is unknown).
 int upload_tconst(es_engine* e, const std::vector<uint32_t>& tc) {
   for (uint32_t f = 0; f < e->F; ++f)
     e->mask_info[kMaskInfo * f + 7] = tc[e->mask_src[f]] % e->mask_mod[f];
   for (uint32_t j = 0; j < e->n_slow; ++j) e->slow_info[kSlowInfo * j + 5] = tc[e->slow_order[j]];
+  e->floor = (e->t_known && e->t == 0) ? (uint64_t)e->k + 2 : 0;
   ES_HIP(hipMemcpyAsync(e->d_mask_info, e->mask_info.data(), e->mask_info.size() * 4,
                         hipMemcpyHostToDevice, e->stream));
   ES_HIP(hipMemcpyAsync(e->d_slow_info, e->slow_info.data(), e->slow_info.size() * 4,
                         hipMemcpyHostToDevice, e->stream));
+  ES_HIP(hipMemcpyAsync(e->d_count + 2, &e->floor, 8, hipMemcpyHostToDevice, e->stream));
   ES_HIP(hipStreamSynchronize(e->stream));  // host records may change right after
   return ES_OK;
 }
@@ -991,7 +554,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   const size_t o_slow = bl.put(e->slow_info.data(), e->slow_info.size() * 4);
   const size_t o_sbits = bl.put(fbits.data(), fbits.size() * 4);
   const size_t o_cold = bl.put(nullptr, sizeof(ColdPlan));
-  const size_t o_cnt = bl.put(nullptr, 8);
+  const size_t o_cnt = bl.put(nullptr, 24);
 
   ES_HIP(hipGetDevice(&e->device));
   hipDeviceProp_t prop;
@@ -1027,7 +590,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->max_grid = 1u << 30;  // one strip per thread; the dispatcher balances the tail
   ES_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   ES_HIP(hipMalloc(&e->dblob, bl.bytes.size()));
-  ES_HIP(hipMalloc(&e->d_out, std::max<uint64_t>(e->out_cap, 1) * 8));
+  if (e->out_cap > (1ull << 32)) return fail(ES_ERR_LIMIT, "out_cap above 2^32 survivors");
+  ES_HIP(hipMalloc(&e->d_out, std::max<uint64_t>(e->out_cap, 1) * 24));
   uint8_t* base = (uint8_t*)e->dblob;
   ColdPlan cold{};
   cold.M0_lo = (uint64_t)e->M0;
@@ -1044,6 +608,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   cold.slow_bits = (const uint32_t*)(base + o_sbits);
   cold.out = e->d_out;
   cold.out_n = (unsigned long long*)(base + o_cnt);
+  cold.out_s = e->d_out + std::max<uint64_t>(e->out_cap, 1);
   cold.out_cap = e->out_cap;
   e->cold = cold;
   std::memcpy(bl.bytes.data() + o_cold, &cold, sizeof(cold));
@@ -1078,7 +643,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->launch_outer = std::max<uint64_t>(1, ((uint64_t)1 << (d->launch_log2 ? d->launch_log2 : 37)) / r0);
   e->ev_pool.resize(512);
   for (auto& ev : e->ev_pool) ES_HIP(hipEventCreate(&ev));
-  ES_HIP(hipMemsetAsync(e->d_count, 0, 8, e->stream));
+  ES_HIP(hipMemsetAsync(e->d_count, 0, 24, e->stream));
   std::vector<uint32_t> tc0(std::max<uint32_t>(d->n_filters, 1), 0);
   const int rc = upload_tconst(e, tc0);
   if (rc) return rc;
@@ -1122,6 +687,7 @@ int es_engine_set_tconst(es_engine* e, const uint32_t* tconst, uint32_t n) {
   tc.resize(std::max<uint32_t>(n, 1), 0);
   for (uint32_t j = 0; j < n; ++j)
     if (tc[j] >= e->filt_q[j]) return fail(ES_ERR_INVALID, "tconst entry not reduced mod q");
+  e->t_known = false;
   return upload_tconst(e, tc);
 }
 
@@ -1133,13 +699,14 @@ int es_engine_set_block(es_engine* e, uint64_t t) {
     tc[j] = (uint32_t)(((uint64_t)(t % q) * mod_u128(e->M, q)) % q);
   }
   e->t = t;
+  e->t_known = true;
   return upload_tconst(e, tc);
 }
 
 int es_engine_reset(es_engine* e) {
   if (!e) return fail(ES_ERR_INVALID, "null engine");
   e->count_valid = false;
-  ES_HIP(hipMemsetAsync(e->d_count, 0, 8, e->stream));
+  ES_HIP(hipMemsetAsync(e->d_count, 0, 16, e->stream));  // count and minimum, not the floor
   return ES_OK;
 }
 
@@ -1203,45 +770,35 @@ int es_engine_read(es_engine* e, uint64_t offset, uint64_t count, uint64_t* ids)
 int es_engine_collect(es_engine* e, uint64_t* ids, uint64_t cap, uint64_t* n, uint64_t min_s[2],
                       int* has_min) {
   if (!e || !n) return fail(ES_ERR_INVALID, "null argument");
-  uint64_t c = 0;
-  int rc = es_engine_count(e, &c);
-  if (rc) return rc;
-  *n = c;
   if (has_min) *has_min = 0;
+  if ((min_s || has_min) && !e->t_known)
+    return fail(ES_ERR_INVALID, "the minimum needs the block index: set it with "
+                                "es_engine_set_block (es_engine_set_tconst leaves it unknown)");
+  // the counter and the minimum's slot, read together after the stream drains
+  unsigned long long ctr[2] = {0, 0};
+  ES_HIP(hipMemcpyAsync(ctr, e->d_count, 16, hipMemcpyDeviceToHost, e->stream));
+  ES_HIP(hipStreamSynchronize(e->stream));
+  e->last_count = ctr[0];
+  e->count_valid = true;
+  const uint64_t c = ctr[0];
+  *n = c;
   if (c > e->out_cap) return fail(ES_ERR_OVERFLOW, "survivor buffer overflow");
-  std::vector<uint64_t> buf(c);
-  if (c) {
-    rc = es_engine_read(e, 0, c, buf.data());
+  if (ids && c && cap) {
+    const int rc = es_engine_read(e, 0, std::min<uint64_t>(c, cap), ids);
     if (rc) return rc;
   }
-  if (ids) std::memcpy(ids, buf.data(), std::min<uint64_t>(c, cap) * 8);
-  // least s with t*M + s > k + 1
-  bool have = false;
-  u128 best = 0;
-  for (uint64_t v : buf) {
-    uint64_t idx = v;
-    u128 s = 0;
-    for (uint32_t jj = 0; jj < e->w; ++jj) {
-      // idx digits in the reference order: with a lead block the kernel's rings 1..j come
-      // first, then its ring 0
-      const uint32_t lj = e->lead_rings;
-      const uint32_t j = lj && jj <= lj ? (jj < lj ? jj + 1 : 0) : jj;
-      const uint64_t dj = idx % e->radix[j];
-      idx /= e->radix[j];
-      s += e->terms[e->term_off[j] + dj];
-      if (s >= e->M) s -= e->M;
+  if (ctr[1] && (min_s || has_min)) {
+    // the record of the least residue s (n = t*M + s > k + 1), reduced on the device
+    uint64_t rec[2] = {0, 0};
+    ES_HIP(hipMemcpyAsync(rec, e->cold.out_s + 2 * (ctr[1] - 1), 16, hipMemcpyDeviceToHost,
+                          e->stream));
+    ES_HIP(hipStreamSynchronize(e->stream));
+    if (min_s) {
+      min_s[0] = rec[0];
+      min_s[1] = rec[1];
     }
-    if (e->t == 0 && s <= (u128)e->k + 1) continue;
-    if (!have || s < best) {
-      best = s;
-      have = true;
-    }
+    if (has_min) *has_min = 1;
   }
-  if (have && min_s) {
-    min_s[0] = (uint64_t)best;
-    min_s[1] = (uint64_t)(best >> 64);
-  }
-  if (has_min) *has_min = have ? 1 : 0;
   return ES_OK;
 }
 
@@ -1303,10 +860,14 @@ struct es_batch {
 // nodes).
 __global__ void es_zero_counters_kernel(unsigned long long* const* __restrict__ ptrs, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) *ptrs[i] = 0ull;
+  if (i < n) {
+    ptrs[i][0] = 0ull;  // survivors
+    ptrs[i][1] = 0ull;  // slot of the minimum
+  }
 }
 
 // Block constants of a job: the mask and slow-filter records of engine e with t*M mod q.
+static uint64_t job_floor(const es_engine* e, uint64_t t) { return t == 0 ? (uint64_t)e->k + 2 : 0; }
 static void job_records(const es_engine* e, uint64_t t, std::vector<uint32_t>& mi,
                         std::vector<uint32_t>& si) {
   std::vector<uint32_t> tc(std::max<uint32_t>(e->n_filters, 1), 0);
@@ -1382,14 +943,17 @@ int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_
   j.o_mi = o_mi;
   j.o_si = o_si;
   const size_t o_cold = bl.put(nullptr, sizeof(ColdPlan));
-  const size_t o_cnt = bl.put(nullptr, 8);
+  const size_t o_cnt = bl.put(nullptr, 24);
+  const uint64_t fl = job_floor(e, t);
+  std::memcpy(bl.bytes.data() + o_cnt + 16, &fl, 8);
   ES_HIP(hipMalloc(&j.dmem, bl.bytes.size()));
-  ES_HIP(hipMalloc((void**)&j.d_out, j.cap * 8));
+  ES_HIP(hipMalloc((void**)&j.d_out, j.cap * 24));
   uint8_t* base = (uint8_t*)j.dmem;
   ColdPlan cold = e->cold;
   cold.slow = (const uint32_t*)(base + o_si);
   cold.out = j.d_out;
   cold.out_n = (unsigned long long*)(base + o_cnt);
+  cold.out_s = j.d_out + j.cap;
   cold.out_cap = j.cap;
   std::memcpy(bl.bytes.data() + o_cold, &cold, sizeof(cold));
   ES_HIP(hipMemcpy(j.dmem, bl.bytes.data(), bl.bytes.size(), hipMemcpyHostToDevice));
@@ -1408,14 +972,17 @@ int es_batch_set_blocks(es_batch* b, const uint64_t* t, uint32_t n_jobs) {
     return fail(ES_ERR_INVALID, "one block per job expected");
   // all records staged in one host buffer, copied in order, one synchronisation
   std::vector<std::vector<uint32_t>> stage(2 * (size_t)n_jobs);
+  std::vector<uint64_t> floors(n_jobs);
   for (uint32_t i = 0; i < n_jobs; ++i) {
     es_batch::Job& j = b->jobs[i];
     job_records(j.e, t[i], stage[2 * i], stage[2 * i + 1]);
+    floors[i] = job_floor(j.e, t[i]);
     uint8_t* base = (uint8_t*)j.dmem;
     ES_HIP(hipMemcpyAsync(base + j.o_mi, stage[2 * i].data(), stage[2 * i].size() * 4,
                           hipMemcpyHostToDevice, b->stream));
     ES_HIP(hipMemcpyAsync(base + j.o_si, stage[2 * i + 1].data(), stage[2 * i + 1].size() * 4,
                           hipMemcpyHostToDevice, b->stream));
+    ES_HIP(hipMemcpyAsync(j.d_count + 2, &floors[i], 8, hipMemcpyHostToDevice, b->stream));
   }
   ES_HIP(hipStreamSynchronize(b->stream));
   for (uint32_t i = 0; i < n_jobs; ++i) b->jobs[i].t = t[i];
@@ -1428,9 +995,11 @@ int es_batch_set_block(es_batch* b, uint32_t job, uint64_t t) {
   std::vector<uint32_t> mi, si;
   job_records(j.e, t, mi, si);
   uint8_t* base = (uint8_t*)j.dmem;
+  const uint64_t fl = job_floor(j.e, t);
   // ordered after earlier work on the batch's stream (replays read these records)
   ES_HIP(hipMemcpyAsync(base + j.o_mi, mi.data(), mi.size() * 4, hipMemcpyHostToDevice, b->stream));
   ES_HIP(hipMemcpyAsync(base + j.o_si, si.data(), si.size() * 4, hipMemcpyHostToDevice, b->stream));
+  ES_HIP(hipMemcpyAsync(j.d_count + 2, &fl, 8, hipMemcpyHostToDevice, b->stream));
   ES_HIP(hipStreamSynchronize(b->stream));
   j.t = t;
   return ES_OK;
@@ -1579,7 +1148,7 @@ int es_batch_run(es_batch* b, double* ms, uint32_t* launches) {
   std::vector<BatchLaunch> runs;
   int rc = prepare_batch(b, runs);
   if (rc) return rc;
-  for (auto& j : b->jobs) ES_HIP(hipMemsetAsync(j.d_count, 0, 8, b->stream));
+  for (auto& j : b->jobs) ES_HIP(hipMemsetAsync(j.d_count, 0, 16, b->stream));
   ES_HIP(hipEventRecord(b->ev0, b->stream));
   if ((rc = launch_batch(b, runs, false))) return rc;
   ES_HIP(hipEventRecord(b->ev1, b->stream));

@@ -101,6 +101,8 @@ def lib() -> C.CDLL:
         "es_mitm_read": (C.c_int, [eng, u64p, C.c_uint64]),
         "es_mitm_last": (C.c_int, [eng, C.POINTER(C.c_double), u64p]),
         "es_calib_valu": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), u64p]),
+        "es_calib_lds": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                   u64p]),
         "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
         "es_mlp_destroy": (C.c_int, [eng]),
         "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),
@@ -151,5 +153,12 @@ def device_query() -> dict:
     fr, tot = C.c_uint64(0), C.c_uint64(0)
     name = C.create_string_buffer(256)
     check(lib().es_device_query(C.byref(n), C.byref(fr), C.byref(tot), name, 256))
-    return {"devices": n.value, "free_mib": fr.value, "total_mib": tot.value,
-            "name": name.value.decode(errors="replace")}
+    text = name.value.decode(errors="replace")
+    cus = None
+    if text.endswith(" CUs)"):            # "<name> (<arch>, <N> CUs)"
+        try:
+            cus = int(text.rsplit(",", 1)[1].split()[0])
+        except (IndexError, ValueError):
+            cus = None
+    return {"devices": n.value, "free_mib": fr.value, "total_mib": tot.value, "name": text,
+            "cus": cus}

@@ -475,11 +475,13 @@ class GpuSieve:
         self.launches += 1
 
     def set_block(self, t: int) -> None:
+        """Block constants of block t, computed by the library (which then also knows the
+        block: in block 0 the minimum leaves out n <= k+1)."""
         if self._tconst_block == t:            # already uploaded for this block
             return
         for j, q in enumerate(self.rest_q):
             self.tconst_h[j] = ((t % q) * self.rest_Mmodq[j]) % q
-        self.tconst.set(self.tconst_h)
+        nat.check(nat.lib().es_engine_set_block(self._h, int(t)))
         self._tconst_block = t
 
     def scan_block(self, t: int) -> list[int]:
@@ -566,6 +568,23 @@ class GpuSieve:
         if has.value:
             min_n = t * self.plan.M + (int(ms[1]) << 64 | int(ms[0]))
         return [int(x) for x in ids[:cnt]], min_n
+
+    def lds_bytes_per_position(self) -> float:
+        """Algorithmic LDS bytes the sieve kernel reads per residue position: per outer index
+        (r0 positions of the kernel's ring 0) each lane reads one mask row of 8 bytes per
+        ring-0 word from each of the masks staged in LDS, and its carry entry -- one byte of the
+        j(h) table (one-word rows, q0 <= 256, every mask in LDS), a row of one 4-byte word per
+        ring-0 word (two words), or a u16 j(h) (wider rows).  The candidate queue (a few
+        entries per thousand outer indices) and the per-workgroup table staging are left out."""
+        info = self.info
+        W, F, G = int(info.words), int(info.n_masks_used), int(info.global_masks)
+        if W == 1 and info.q0 <= 256 and G == 0:
+            carry = 1
+        elif W <= 2:
+            carry = 4 * W
+        else:
+            carry = 2
+        return ((F - G) * 8 * W + carry) / int(info.r0)
 
     def kernel_time(self):
         ms = C.c_double(0)

@@ -1,8 +1,8 @@
 /* es_calib.h -- C ABI of the measurement calibration kernels, part of libes_sieve.so.
  *
- * No counterpart in the reference: the reference prices nothing against hardware peaks.  The
- * sieve's roofline is quoted against the INT32 VALU peak of the device; this kernel measures
- * that peak on the device it runs on, so the quoted peak is confirmed rather than assumed. */
+ * No counterpart in the reference: the reference prices nothing against hardware peaks.  These
+ * kernels measure, on the device they run on, the peaks the engine's kernels are priced
+ * against: the INT32 VALU add rate and the LDS 8-byte read rate. */
 #ifndef ES_CALIB_H
 #define ES_CALIB_H
 
@@ -17,6 +17,14 @@ extern "C" {
  * kernel time (ms, HIP events) and the 32-bit integer adds executed (lanes x adds).  A
  * checksum of the chains is written so the work cannot be removed. */
 int es_calib_valu(uint32_t blocks, uint32_t iters, double* ms, uint64_t* ops);
+
+/* Runs `blocks` workgroups of 1024 threads; each thread makes `iters` rounds of 8 reads of an
+ * 8-byte row (ds_read_b64) of a 32 KiB LDS table, at its lane's offset plus a wave-uniform
+ * step: mode 0 gives a wave 64 consecutive rows (conflict-free: the LDS read peak), mode 1 a
+ * fixed pseudo-random row per lane (the sieve's mask-row pattern).  Returns the kernel time
+ * (ms, HIP events) and the bytes read from LDS.  The sieve's roofline is priced against the
+ * mode-0 rate. */
+int es_calib_lds(uint32_t blocks, uint32_t iters, uint32_t mode, double* ms, uint64_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -123,7 +123,8 @@ int es_engine_destroy(es_engine* e);
 int es_engine_info_get(const es_engine* e, es_engine_info* info);
 
 /* Per-block constants tconst[j] = (t*M) mod q_j for every filter j, as the reference drivers
- * compute and upload them (sieve/gpu_sieve_v9.py:512-514, harness/run_k_masks.py:131-134). */
+ * compute and upload them (sieve/gpu_sieve_v9.py:512-514, harness/run_k_masks.py:131-134).
+ * The block index itself stays unknown to the engine (see es_engine_collect). */
 int es_engine_set_tconst(es_engine* e, const uint32_t* tconst, uint32_t n);
 /* Same, computed by the library from the block index t. */
 int es_engine_set_block(es_engine* e, uint64_t t);
@@ -141,7 +142,11 @@ int es_engine_count(es_engine* e, uint64_t* n);
 int es_engine_read(es_engine* e, uint64_t offset, uint64_t count, uint64_t* ids);
 /* Synchronise; copy up to cap survivor IDs into ids, set *n to the count, and set
  * min_s = {lo, hi} to the least residue s (n = tM + s) over survivors with n > k+1
- * (*has_min = 0 if there is none).  Returns ES_ERR_OVERFLOW if the buffer overflowed. */
+ * (*has_min = 0 if there is none).  Returns ES_ERR_OVERFLOW if the buffer overflowed.
+ * The minimum is kept on the device while the kernels run (each survivor records s beside
+ * its ID; one compare-and-swap per wave keeps the slot of the least s), so only the IDs and
+ * one 16-byte record are copied back.  It needs the block index: after es_engine_set_tconst
+ * (block unknown) asking for min_s or has_min fails with ES_ERR_INVALID. */
 int es_engine_collect(es_engine* e, uint64_t* ids, uint64_t cap, uint64_t* n,
                       uint64_t min_s[2], int* has_min);
 int es_engine_sync(es_engine* e);

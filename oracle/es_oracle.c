@@ -121,6 +121,9 @@ typedef struct {
   uint64_t lo, hi;
   uint64_t* out;
   uint64_t cap, n;
+  uint64_t t;
+  u128 best;  /* least s over survivors with n = t*M + s > k + 1 (when have) */
+  int have;
 } eso_job;
 
 static void* eso_worker(void* arg) {
@@ -145,6 +148,10 @@ static void* eso_worker(void* arg) {
     if (ok) {
       if (J->n < J->cap) J->out[J->n] = idx;
       J->n++;
+      if ((J->t != 0 || s > (u128)P->k + 1) && (!J->have || s < J->best)) {
+        J->best = s;
+        J->have = 1;
+      }
     }
   }
   return NULL;
@@ -152,9 +159,24 @@ static void* eso_worker(void* arg) {
 
 /* Survivors of positions [idx_lo, idx_hi) of block t, ascending.  Returns the count (which may
  * exceed cap; only cap IDs are written), or -1 on an invalid plan. */
+/* As eso_scan, and min_out = {lo, hi, have}: the least s (n = t*M + s) over all survivors with
+ * n > k + 1 (ref/erdos_ref.py:121-128: the minimum of a block counts only n > k + 1). */
+int64_t eso_scan_min(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
+                     uint32_t nf, const uint32_t* filt_p, uint64_t t, uint64_t idx_lo,
+                     uint64_t idx_hi, uint32_t nthreads, uint64_t* out, uint64_t cap,
+                     uint64_t min_out[3]);
+
 int64_t eso_scan(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
                  uint32_t nf, const uint32_t* filt_p, uint64_t t, uint64_t idx_lo,
                  uint64_t idx_hi, uint32_t nthreads, uint64_t* out, uint64_t cap) {
+  uint64_t m[3];
+  return eso_scan_min(k, w, ring_p, ring_T, nf, filt_p, t, idx_lo, idx_hi, nthreads, out, cap, m);
+}
+
+int64_t eso_scan_min(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t* ring_T,
+                     uint32_t nf, const uint32_t* filt_p, uint64_t t, uint64_t idx_lo,
+                     uint64_t idx_hi, uint32_t nthreads, uint64_t* out, uint64_t cap,
+                     uint64_t min_out[3]) {
   eso_plan* P = eso_build(k, w, ring_p, ring_T, nf, filt_p, t);
   if (!P) return -1;
   if (nthreads < 1) nthreads = 1;
@@ -168,12 +190,19 @@ int64_t eso_scan(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t*
     jobs[i].lo = idx_lo + span * i / nthreads;
     jobs[i].hi = idx_lo + span * (i + 1) / nthreads;
     jobs[i].cap = cap;
+    jobs[i].t = t;
     jobs[i].out = (uint64_t*)malloc(sizeof(uint64_t) * (cap ? cap : 1));
     pthread_create(&th[i], NULL, eso_worker, &jobs[i]);
   }
   uint64_t total = 0;
+  u128 best = 0;
+  int have = 0;
   for (uint32_t i = 0; i < nthreads; ++i) {
     pthread_join(th[i], NULL);
+    if (jobs[i].have && (!have || jobs[i].best < best)) {
+      best = jobs[i].best;
+      have = 1;
+    }
     const uint64_t take = jobs[i].n < cap ? jobs[i].n : cap;
     for (uint64_t j = 0; j < take && total + j < cap; ++j) out[total + j] = jobs[i].out[j];
     total += jobs[i].n;
@@ -182,6 +211,9 @@ int64_t eso_scan(uint32_t k, uint32_t w, const uint32_t* ring_p, const uint32_t*
   free(jobs);
   free(th);
   eso_free(P);
+  min_out[0] = (uint64_t)best;
+  min_out[1] = (uint64_t)(best >> 64);
+  min_out[2] = (uint64_t)have;
   return (int64_t)total;
 }
 

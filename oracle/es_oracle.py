@@ -187,6 +187,10 @@ def lib():
         L.eso_scan.restype = C.c_int64
         L.eso_scan.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, u32p, C.c_uint64,
                                C.c_uint64, C.c_uint64, C.c_uint32, u64p, C.c_uint64]
+        L.eso_scan_min.restype = C.c_int64
+        L.eso_scan_min.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, u32p,
+                                   C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, u64p,
+                                   C.c_uint64, u64p]
         L.eso_geometry.restype = C.c_int
         L.eso_geometry.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, u64p, u64p, u32p]
         L.eso_idx_to_s.restype = C.c_int
@@ -223,6 +227,28 @@ def c_scan(k: int, wheel_ring_order, filt_primes, t: int, idx_lo: int, idx_hi: i
     if n < 0:
         raise ValueError("oracle rejected the plan")
     return int(n), out[:min(n, cap)].copy()
+
+
+def c_scan_min(k: int, wheel_ring_order, filt_primes, t: int, idx_lo: int, idx_hi: int,
+               threads: int = 8, cap: int = 1 << 22):
+    """As c_scan, plus the least survivor n = t*M + s > k+1 of the window (None if none):
+    (count, ids, min_n)."""
+    import numpy as np
+    L = lib()
+    rp = _u32([p for p, _ in wheel_ring_order])
+    rT = _u32([T for _, T in wheel_ring_order])
+    fp = _u32(list(filt_primes))
+    out = np.zeros(max(1, cap), dtype=np.uint64)
+    m = (C.c_uint64 * 3)()
+    n = L.eso_scan_min(k, len(wheel_ring_order), rp, rT, len(filt_primes), fp, t, idx_lo,
+                       idx_hi, threads, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap, m)
+    if n < 0:
+        raise ValueError("oracle rejected the plan")
+    mn = None
+    if m[2]:
+        M, _, _ = c_geometry(k, wheel_ring_order)
+        mn = t * M + ((int(m[1]) << 64) | int(m[0]))
+    return int(n), out[:min(n, cap)].copy(), mn
 
 
 def c_geometry(k: int, wheel_ring_order):

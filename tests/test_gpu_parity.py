@@ -44,15 +44,12 @@ def window_check(k, wheel, t, lo, hi, limit=None, **tune):
         f"strip={eng.strip_len} r0={eng.r0}")
     ids, mn = eng.scan_window(t, lo, hi)
     got = np.sort(np.asarray(ids, dtype=np.uint64))
-    n, want = O.c_scan(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
+    n, want, want_min = O.c_scan_min(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
     assert n == len(want)
     assert len(got) == n, f"count {len(got)} != oracle {n}"
     assert np.array_equal(got, want)
-    # minimum over n > k+1 agrees with the oracle's idx -> n map
-    ns = [O.c_idx_to_n(k, plan.wheel, int(i), t) for i in want[:2000]] if n <= 2000 else None
-    if ns is not None:
-        valid = [x for x in ns if x > k + 1]
-        assert mn == (min(valid) if valid else None)
+    # the device-side minimum over n > k+1 equals the oracle's
+    assert mn == want_min, f"minimum {mn} != oracle {want_min}"
     return n
 
 
@@ -66,6 +63,41 @@ def test_config1_unaligned_windows():
         lo = rng.randrange(0, 230947200 - (1 << 20))
         hi = lo + rng.randrange(1, 1 << 20)
         window_check(30, C1_WHEEL, rng.randrange(0, 50), lo, hi)
+
+
+def test_later_block_minimum_counts_every_residue():
+    """In block t >= 1 every survivor counts for the minimum, s <= k+1 included: block 14 of the
+    C1 wheel starts with a survivor at idx 0, n = 14 M + 30."""
+    eng, plan = engine(30, C1_WHEEL)
+    assert window_check(30, C1_WHEEL, 14, 0, 1 << 16) > 0
+    _, mn = eng.scan_window(14, 0, 1 << 16)
+    assert mn == 14 * plan.M + 30
+    _, mn0 = eng.scan_window(0, 0, 1 << 16)   # block 0: s <= 31 left out
+    assert mn0 is None or mn0 > 31
+
+
+def test_minimum_needs_the_block():
+    """Constants uploaded as tconst (block unknown): IDs still collect, the minimum is refused."""
+    import ctypes as C
+    import es_native as nat
+    eng, plan = engine(30, C1_WHEEL)
+    say("case k=30 tconst upload, minimum refused")
+    tc = np.asarray([((3 % q) * (plan.M % q)) % q for q in eng.rest_q], dtype=np.uint32)
+    L = nat.lib()
+    nat.check(L.es_engine_set_tconst(eng._h, nat.ptr32(tc), len(tc)))
+    nat.check(L.es_engine_reset(eng._h))
+    nat.check(L.es_engine_scan_window(eng._h, 0, 1 << 16))
+    n = C.c_uint64(0)
+    ids = np.zeros(1 << 16, dtype=np.uint64)
+    nat.check(L.es_engine_collect(eng._h, nat.ptr64(ids), 1 << 16, C.byref(n), None, None))
+    want_n, want = O.c_scan(30, plan.wheel, plan.rest, 3, 0, 1 << 16, threads=4)
+    assert n.value == want_n and np.array_equal(np.sort(ids[:n.value]), want)
+    ms = np.zeros(2, dtype=np.uint64)
+    has = C.c_int(0)
+    rc = L.es_engine_collect(eng._h, nat.ptr64(ids), 1 << 16, C.byref(n), nat.ptr64(ms), C.byref(has))
+    assert rc == nat.ES_ERR_INVALID and b"block index" in L.es_last_error()
+    _, mn = eng.scan_window(3, 0, 1 << 16)     # set_block makes the block known again
+    assert mn == O.c_scan_min(30, plan.wheel, plan.rest, 3, 0, 1 << 16, threads=4)[2]
 
 
 def test_config1_full_block_min():

@@ -188,3 +188,22 @@ def test_word_mask_tables_definition(k):
         for i, tv in enumerate(terms):
             bit = (tab[ph][i // 32] >> (i % 32)) & 1
             assert bit == int((ph + tv) % q in set(res))
+
+
+def test_scan_min_pinned():
+    """The oracle's window minimum: g(30) in block 0 of the C1 wheel (s <= k+1 left out there),
+    every later block's least survivor counted whatever its residue, and both equal to the
+    Python restatement's idx -> n map over the survivor list."""
+    import es_plan as EP
+    plan = EP.Plan(30, None, [[2, 5], [3, 4], [5, 3], [7, 2], [11, 2], [13, 2]])
+    wm = O.WheelMap(30, [tuple(x) for x in plan.wheel])
+    n, ids, mn = O.c_scan_min(30, plan.wheel, plan.rest, 0, 0, 1 << 24, threads=8)
+    assert mn == 58782 and n == len(ids) > 0
+    # block 14 starts with a survivor at idx 0: n = 14 M + 30, counted although s = 30 <= k+1
+    n14, ids14, mn14 = O.c_scan_min(30, plan.wheel, plan.rest, 14, 0, 1 << 16, threads=4)
+    assert ids14[0] == 0 and mn14 == 14 * wm.M + 30
+    for t, lo in ((0, 0), (3, 5_000_000), (14, 0)):
+        n, ids, mn = O.c_scan_min(30, plan.wheel, plan.rest, t, lo, lo + (1 << 14), threads=2)
+        ns = [wm.idx_to_n(int(i), t) for i in ids]
+        valid = [x for x in ns if x > 31]
+        assert mn == (min(valid) if valid else None)

@@ -571,11 +571,8 @@ def bench_surrogate(a) -> int:
     import es_plan
     import es_surrogate as S
     import gpu_sieve_mi355x as G
-    bf = {}
-    for line in open(os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt")):
-        parts = line.split()
-        if len(parts) == 2 and not line.startswith("#"):
-            bf[int(parts[0])] = int(parts[1])
+    import es_data
+    bf = es_data.known_terms()
     ks = list(range(350, 401, a.k_step))
     SHORTLISTS = (1, 2, 4, 8, 32)
     t0 = time.perf_counter()
@@ -677,11 +674,8 @@ def bench_mitm(a) -> int:
     import es_mitm as MM
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import es_oracle as O
-    bf = {}
-    for line in open(os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt")):
-        parts = line.split()
-        if len(parts) == 2:
-            bf[int(parts[0])] = int(parts[1])
+    import es_data
+    bf = es_data.known_terms()
     k = a.mitm_k
     t0 = time.perf_counter()
     s = MM.MitmSieve(k, mem_gb=a.mitm_mem)

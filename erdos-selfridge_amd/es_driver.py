@@ -1,14 +1,21 @@
-"""Crash-resumable block driver for the MI355X engine.
+"""Crash-resumable block driver: runs blocks t = 0, 1, ... of one wheel through an engine module
+until a block holds a valid survivor, checkpointing inside blocks.
 
-Counterpart of harness/run_k_masks.py:74-224 (and harness/run_k.py:54-186): it runs blocks
-t = 0, 1, ... of one plan through any engine with the GpuSieve contract, checkpoints inside a
-block (atomic write + fsync), resumes from the checkpoint (refusing a plan or engine revision
-mismatch), pauses on SIGTERM/SIGINT or after --max-launches at a launch boundary, and records a
-block_audit row per completed block with the survivor hash recipe of harness/run_k.py:38-44.
+Role of harness/run_k_masks.py:74-224 (and harness/run_k.py:54-186) for any module with the
+GpuSieve contract (gpu_sieve_mi355x here).  Shared with the reference only where callers and
+records depend on it: the checkpoint and result keys, the block_audit row schema, the survivor
+hash recipe (harness/run_k.py:38-44), the status strings and the CLI flags.
 
-Block results are exact whatever the interruption pattern: survivor IDs are block-absolute,
-the checkpoint stores those found before unit_done, and resuming re-scans only units >=
-unit_done, so the block's survivor set is reproduced exactly.
+Design.  A `Checkpoint` owns the persisted state (one JSON object, written atomically with
+fsync) and knows when it may be resumed: same k, wheel, M, R, engine module and engine source
+revision, and internally consistent (units done within the block, survivor IDs unique and all
+below the scanned prefix).  `Budget` decides, before each launch, whether the run must stop
+(signal, launch budget, deadline).  `scan_block` advances the current block launch by launch
+and records each launch's survivor IDs before the unit counter moves past it, so a checkpoint
+never claims a unit whose survivors it lacks.  `conclude_block` turns a finished block's IDs
+into n, applies the referee, and writes the audit row.  Survivor IDs are block-absolute, so
+resuming re-scans exactly the units not yet recorded and the block's survivor set is the same
+whatever the interruption pattern.
 """
 from __future__ import annotations
 
@@ -22,7 +29,7 @@ import signal
 import sys
 import time
 from pathlib import Path
-from typing import Any
+from typing import Any, Callable
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -30,203 +37,263 @@ if HERE not in sys.path:
     sys.path.insert(0, HERE)
 
 import es_plan  # noqa: E402
+import es_data  # noqa: E402
 
 SURV_HASH_RECIPE = ("sha256 of the UTF-8 bytes of '\\n'.join(decimal(n) for n in sorted(survivors)), "
                     "no trailing newline; the empty list hashes to sha256(b'')")
+MISMATCH = "Checkpoint plan/engine revision mismatch; explicit qualified migration required"
 
 STOP_REQUESTED = False
 
 
 def request_stop(signum: int, frame: Any) -> None:
+    """Signal handler: the run pauses at the next launch boundary."""
     global STOP_REQUESTED
     STOP_REQUESTED = True
 
 
 def survivor_hash(ns) -> str:
-    """sha256 of '\\n'.join(sorted decimal n), reproducible by a third party."""
-    return hashlib.sha256("\n".join(str(int(n)) for n in sorted(ns)).encode()).hexdigest()
+    text = "\n".join(str(int(n)) for n in sorted(ns))
+    return hashlib.sha256(text.encode()).hexdigest()
 
 
 def engine_revision(engine_module) -> dict[str, str]:
-    """Content hashes of the engine's sources and its native library."""
-    paths = [os.path.join(HERE, "es_plan.py"), os.path.join(HERE, "es_driver.py"),
-             os.path.join(HERE, "csrc", "es_sieve.hip")]
-    mod_file = getattr(engine_module, "__file__", None)
-    if mod_file:
-        paths.append(mod_file)
-    out = {}
-    for p in paths:
-        if os.path.exists(p):
-            out[os.path.relpath(p, ROOT)] = hashlib.sha256(Path(p).read_bytes()).hexdigest()
-    return out
+    """Content hashes of what decides the survivor sets: the engine module, the planner, this
+    driver, and (for the HIP engine) the kernel sources."""
+    files = [Path(HERE) / "es_plan.py", Path(HERE) / "es_driver.py"]
+    src = getattr(engine_module, "__file__", None)
+    if src:
+        files.append(Path(src))
+    if getattr(engine_module, "__name__", "") == "gpu_sieve_mi355x":
+        files += sorted((Path(HERE) / "csrc").glob("es_sieve*"))
+    rev = {}
+    for f in files:
+        if f.exists():
+            key = os.path.relpath(f, ROOT) if str(f).startswith(ROOT) else f.name
+            rev[key] = hashlib.sha256(f.read_bytes()).hexdigest()
+    return rev
 
 
-def atomic_write(path: str, obj: dict) -> None:
-    tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(obj, f)
-        f.flush()
-        os.fsync(f.fileno())
-    os.replace(tmp, path)
-    parent = os.open(os.path.dirname(os.path.abspath(path)), os.O_DIRECTORY)
-    try:
-        os.fsync(parent)
-    finally:
-        os.close(parent)
+class Checkpoint:
+    """The persisted state of one (k, wheel, engine) search."""
+
+    def __init__(self, path: str, state: dict):
+        self.path = path
+        self.s = state
+
+    @classmethod
+    def fresh(cls, path: str, k: int, plan, engine: str, revision: dict) -> "Checkpoint":
+        return cls(path, {
+            "engine_revision": revision, "k": k, "engine": engine,
+            "wheel": [[p, T] for p, T in plan.wheel], "M": str(plan.M), "R": plan.R,
+            "t": 0, "unit_done": 0, "idx_done": 0, "survivor_idx": [], "residues_checked": 0,
+            "wall_accum": 0.0, "restarts": 0, "block_audit": []})
+
+    def identity(self) -> tuple:
+        s = self.s
+        return (s.get("k"), s.get("wheel"), s.get("R"), s.get("engine"), s.get("M"),
+                s.get("engine_revision"))
+
+    @classmethod
+    def resume_or_fresh(cls, path: str, k: int, plan, engine: str, revision: dict,
+                        resume: bool) -> "Checkpoint":
+        new = cls.fresh(path, k, plan, engine, revision)
+        if not (resume and os.path.exists(path)):
+            return new
+        with open(path) as fh:
+            old = cls(path, json.load(fh))
+        if old.identity() != new.identity():
+            raise RuntimeError(MISMATCH)
+        old.s.setdefault("block_audit", [])
+        old.s["restarts"] = int(old.s.get("restarts", 0)) + 1
+        return old
+
+    def check(self, units_total: int, unit: int) -> None:
+        """Invariants a resumable state must satisfy (any violation is a corrupt checkpoint)."""
+        s = self.s
+        ids = s["survivor_idx"]
+        problems = [
+            (0 <= s["unit_done"] <= units_total, "units done outside the block"),
+            (s["idx_done"] == s["unit_done"] * unit, "idx_done != unit_done * unit"),
+            (s["residues_checked"] >= s["idx_done"], "fewer residues checked than scanned"),
+            (len(set(ids)) == len(ids), "repeated survivor ID"),
+            (all(0 <= i < s["idx_done"] for i in ids), "survivor ID beyond the scanned prefix"),
+        ]
+        for ok, what in problems:
+            if not ok:
+                raise AssertionError(f"corrupt checkpoint {self.path}: {what}")
+
+    def save(self) -> None:
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(self.s, fh)
+            fh.flush()
+            os.fsync(fh.fileno())
+        os.replace(tmp, self.path)
+        d = os.open(os.path.dirname(os.path.abspath(self.path)), os.O_DIRECTORY)
+        try:
+            os.fsync(d)
+        finally:
+            os.close(d)
+
+    def next_block(self) -> None:
+        self.s.update(t=self.s["t"] + 1, unit_done=0, idx_done=0, survivor_idx=[])
+
+
+class Budget:
+    """Stop conditions checked before every launch: a stop signal or an exhausted launch budget
+    pause the run (PAUSED), an expired deadline ends it (TIMEOUT)."""
+
+    def __init__(self, max_launches: int | None, timeout: float | None, started: float):
+        self.max_launches = max_launches
+        self.deadline = started + timeout if timeout else None
+
+    def verdict(self, launches: int) -> str | None:
+        if STOP_REQUESTED or (self.max_launches is not None and launches >= self.max_launches):
+            return "PAUSED"
+        if self.deadline is not None and time.time() > self.deadline:
+            return "TIMEOUT"
+        return None
+
+
+def scan_block(sieve, ck: Checkpoint, units_total: int, unit: int, chunk: int, budget: Budget,
+               on_launch: Callable[[], None]) -> str | None:
+    """Launch the rest of the current block; returns a stop verdict, or None once every unit
+    of the block has been scanned and its survivors recorded."""
+    s = ck.s
+    while s["unit_done"] < units_total:
+        stop = budget.verdict(sieve.launches)
+        if stop:
+            return stop
+        first = s["unit_done"]
+        count = min(chunk, units_total - first)
+        sieve.out_n.fill(0)
+        sieve._launch(first, count)
+        found = int(sieve.out_n.get()[0])            # waits for the launch
+        if found > sieve.out_cap:
+            raise RuntimeError(f"survivor buffer overflow ({found} > {sieve.out_cap})")
+        if found:
+            s["survivor_idx"] += [int(x) for x in sieve.out[:found].get()]
+        sieve.residues_checked += count * unit
+        s.update(unit_done=first + count, idx_done=(first + count) * unit,
+                 residues_checked=sieve.residues_checked)
+        on_launch()
+    return None
+
+
+def conclude_block(plan, k: int, t: int, ids) -> tuple[dict, list[int], list[int]]:
+    """Audit row of a fully scanned block, its referee-approved n, and the n > k+1 the referee
+    rejects (a disagreement between kernel and referee)."""
+    ns = sorted(plan.idx_to_n(i, t) for i in ids)
+    good = [n for n in ns if es_plan.is_good(n, k)]
+    valid = [n for n in ns if n > k + 1]
+    rejected = sorted(set(valid) - set(good))
+    row = {"t": t, "residues_enumerated": plan.R, "kernel_survivors": len(ns),
+           "after_n_gt_k_plus_1": len(valid), "passed_ref_is_good": len(good),
+           "survivors_n": [str(n) for n in ns] if len(ns) <= 64 else None,
+           "sha256_survivors": survivor_hash(ns)}
+    return row, good, rejected
 
 
 def run(engine_module, k: int, ckpt_path: str, tun: dict | None, wheel, max_blocks: int | None,
         timeout: float | None, log_every: float, ckpt_every: float, min_free_mib: int,
         resume: bool, start_block: int | None = None, max_launches: int | None = None,
-        log=None) -> dict[str, Any]:
-    """One k through engine_module (which provides Plan, GpuSieve, free_vram_mib)."""
-    log = log or (lambda s: print(s, file=sys.stderr, flush=True))
-    t_start = time.time()
+        log: Callable[[str], None] | None = None) -> dict[str, Any]:
+    """Search blocks of one wheel until a valid survivor is found, the block limit is reached,
+    or the run is paused / times out (state saved).  Result keys follow the reference driver."""
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+    started = time.time()
     plan = engine_module.Plan(k, tun, wheel)
-    wheel_json = [[p, T] for p, T in plan.wheel]
     engine = engine_module.__name__
-    rev = engine_revision(engine_module)
-    state: dict[str, Any] = {"engine_revision": rev, "k": k, "engine": engine, "wheel": wheel_json,
-                             "M": str(plan.M), "R": plan.R, "t": 0, "unit_done": 0, "idx_done": 0,
-                             "survivor_idx": [], "residues_checked": 0, "wall_accum": 0.0,
-                             "restarts": 0, "block_audit": []}
-    if resume and os.path.exists(ckpt_path):
-        old = json.load(open(ckpt_path))
-        if (old.get("k") == k and old.get("wheel") == wheel_json and old.get("R") == plan.R
-                and old.get("engine") == engine and old.get("M") == str(plan.M)
-                and old.get("engine_revision") == rev):
-            state = old
-            state.setdefault("block_audit", [])
-            state["restarts"] = state.get("restarts", 0) + 1
-            log(f"# RESUME k={k} engine={engine} block t={state['t']} "
-                f"residues_done={state['idx_done']}/{plan.R} survivors={len(state['survivor_idx'])} "
-                f"restarts={state['restarts']}")
-        else:
-            raise RuntimeError("Checkpoint plan/engine revision mismatch; explicit qualified migration required")
-
-    if start_block is not None and state["t"] == 0 and state["idx_done"] == 0:
-        state["t"] = start_block
-
+    ck = Checkpoint.resume_or_fresh(ckpt_path, k, plan, engine, engine_revision(engine_module),
+                                    resume)
+    s = ck.s
+    if s["restarts"]:
+        log(f"# resumed k={k} at block {s['t']}, {s['idx_done']}/{plan.R} positions, "
+            f"{len(s['survivor_idx'])} survivors, restart {s['restarts']}")
+    if start_block is not None and s["t"] == 0 and s["idx_done"] == 0:
+        s["t"] = int(start_block)
+        log(f"# k={k}: fresh run starting at block {start_block}")
     free = engine_module.free_vram_mib()
     if free is not None and free < min_free_mib:
         return {"k": k, "status": "SKIPPED_LOW_VRAM", "free_mib": free}
 
     sieve = engine_module.GpuSieve(plan)
-    sieve.residues_checked = state["residues_checked"]
-    checked_at_start = sieve.residues_checked
-    r0 = getattr(sieve, "r0", 1)
+    unit = getattr(sieve, "r0", 1)
     units_total = getattr(sieve, "R_outer", plan.R)
-    unit_chunk = getattr(sieve, "ochunk", getattr(sieve, "chunk", 1))
-    assert units_total * r0 == plan.R, "unit accounting does not cover the block exactly"
-    assert 0 <= state["unit_done"] <= units_total
-    assert state["idx_done"] == state["unit_done"] * r0
-    assert state["residues_checked"] >= state["idx_done"]
-    assert len(state["survivor_idx"]) == len(set(state["survivor_idx"]))
-    assert all(0 <= i < state["idx_done"] for i in state["survivor_idx"])
-    state["mask_mode"] = getattr(sieve, "mask_mode", "scalar")
-    last_log = last_ckpt = time.time()
-    t0 = time.time()
-
-    def save() -> None:
-        atomic_write(ckpt_path, state)
+    chunk = getattr(sieve, "ochunk", getattr(sieve, "chunk", 1))
+    assert units_total * unit == plan.R, "launch units do not tile the block"
+    ck.check(units_total, unit)
+    sieve.residues_checked = s["residues_checked"]
+    s["mask_mode"] = getattr(sieve, "mask_mode", None)
+    leg0 = time.time()
+    checked0 = sieve.residues_checked
+    budget = Budget(max_launches, timeout, started)
+    clock = {"log": leg0, "save": leg0}
 
     def wall() -> float:
-        return state.get("wall_accum", 0.0) + (time.time() - t0)
+        return s.get("wall_accum", 0.0) + (time.time() - leg0)
 
-    while max_blocks is None or state["t"] < max_blocks:
-        t = state["t"]
-        for j, q in enumerate(sieve.rest_q):
-            sieve.tconst_h[j] = ((t % q) * sieve.rest_Mmodq[j]) % q
+    def common(status: str) -> dict:
+        return {"k": k, "status": status, "block_audit": s["block_audit"],
+                "surv_hash_recipe": SURV_HASH_RECIPE, "residues_checked": sieve.residues_checked,
+                "wall": wall(), "wall_leg": time.time() - leg0, "restarts": s.get("restarts", 0)}
+
+    def on_launch() -> None:
+        now = time.time()
+        if now - clock["log"] >= log_every:
+            rate = (sieve.residues_checked - checked0) / max(now - leg0, 1e-9)
+            left = (plan.R - s["idx_done"]) / rate if rate else math.inf
+            log(f"k={k} block {s['t']}: {s['idx_done']:.4e}/{plan.R:.4e} positions "
+                f"({100 * s['unit_done'] / units_total:.2f} %), {len(s['survivor_idx'])} survivors, "
+                f"{rate:.3e} positions/s, block done in {left / 3600:.2f} h")
+            clock["log"] = now
+        if now - clock["save"] >= ckpt_every:
+            ck.save()
+            clock["save"] = now
+
+    while max_blocks is None or s["t"] < max_blocks:
+        t = s["t"]
+        sieve.tconst_h[:len(sieve.rest_q)] = [((t % q) * mq) % q
+                                              for q, mq in zip(sieve.rest_q, sieve.rest_Mmodq)]
         sieve.tconst.set(sieve.tconst_h)
-
-        while state["unit_done"] < units_total:
-            if (STOP_REQUESTED or (max_launches is not None and sieve.launches >= max_launches)
-                    or (timeout and time.time() - t_start > timeout)):
-                state["residues_checked"] = sieve.residues_checked
-                state["wall_accum"] = wall()
-                save()
-                return {"k": k, "status": "PAUSED" if STOP_REQUESTED or max_launches is not None else "TIMEOUT",
-                        "block": t, "idx_done": state["idx_done"], "block_audit": state["block_audit"],
-                        "surv_hash_recipe": SURV_HASH_RECIPE, "unit_done": state["unit_done"],
-                        "units_total": units_total, "R": plan.R,
-                        "residues_checked": sieve.residues_checked, "wall": state["wall_accum"],
-                        "wall_leg": time.time() - t0, "restarts": state.get("restarts", 0),
-                        "checkpoint": ckpt_path}
-            ustart = state["unit_done"]
-            ucount = min(unit_chunk, units_total - ustart)
-            sieve.out_n.fill(0)
-            sieve._launch(ustart, ucount)
-            nsv = int(sieve.out_n.get()[0])
-            if nsv > sieve.out_cap:
-                raise RuntimeError(f"survivor buffer overflow ({nsv} > {sieve.out_cap})")
-            if nsv:
-                state["survivor_idx"].extend(int(x) for x in sieve.out[:nsv].get())
-            sieve.residues_checked += ucount * r0
-            state["unit_done"] = ustart + ucount
-            state["idx_done"] = state["unit_done"] * r0
-            state["residues_checked"] = sieve.residues_checked
-            now = time.time()
-            if now - last_log >= log_every:
-                el = now - t0
-                rate = (sieve.residues_checked - checked_at_start) / el if el else 0
-                eta = (plan.R - state["idx_done"]) / rate if rate else float("inf")
-                log(f"k={k} t={t} {100 * state['unit_done'] / units_total:6.2f}%  "
-                    f"idx={state['idx_done']:.6e}/{plan.R:.6e} surv={len(state['survivor_idx'])} "
-                    f"checked={sieve.residues_checked:.4e} rate={rate:.3e}/s "
-                    f"elapsed={el / 3600:.2f}h block_eta={eta / 3600:.2f}h")
-                last_log = now
-            if now - last_ckpt >= ckpt_every:
-                save()
-                last_ckpt = now
-
-        cands = sorted(plan.idx_to_n(i, t) for i in state["survivor_idx"])
-        good = [n for n in cands if es_plan.is_good(n, k)]
-        bad = [n for n in cands if n > k + 1 and n not in good]
-        state["block_audit"].append({
-            "t": t, "residues_enumerated": plan.R, "kernel_survivors": len(cands),
-            "after_n_gt_k_plus_1": sum(1 for n in cands if n > k + 1),
-            "passed_ref_is_good": len(good),
-            "survivors_n": [str(n) for n in cands] if len(cands) <= 64 else None,
-            "sha256_survivors": survivor_hash(cands)})
-        if bad:
-            state["residues_checked"] = sieve.residues_checked
-            save()
-            return {"k": k, "status": "DISAGREEMENT", "block": t, "block_audit": state["block_audit"],
-                    "surv_hash_recipe": SURV_HASH_RECIPE, "kernel_survivors": [str(x) for x in cands[:20]],
-                    "ref_rejected": [str(x) for x in bad[:20]], "residues_checked": sieve.residues_checked,
-                    "wall": wall(), "wall_leg": time.time() - t0}
-        if good:
-            state["residues_checked"] = sieve.residues_checked
-            save()
-            return {"k": k, "status": "FOUND", "g": str(good[0]), "block": t, "engine": engine,
-                    "block_audit": state["block_audit"], "surv_hash_recipe": SURV_HASH_RECIPE,
-                    "blocks_scanned": t + 1, "survivors_in_block": len(state["survivor_idx"]),
-                    "all_good_in_block": [str(x) for x in good],
-                    "residues_checked": sieve.residues_checked, "wall": wall(),
-                    "wall_leg": time.time() - t0, "restarts": state.get("restarts", 0)}
-        log(f"k={k} block t={t} complete: no survivor passed the referee; advancing")
-        state["t"] = t + 1
-        state["unit_done"] = 0
-        state["idx_done"] = 0
-        state["survivor_idx"] = []
-        save()
-
-    return {"k": k, "status": "EXHAUSTED_BLOCKS", "blocks": state["t"], "block_audit": state["block_audit"],
-            "surv_hash_recipe": SURV_HASH_RECIPE, "residues_checked": sieve.residues_checked,
-            "wall": wall(), "wall_leg": time.time() - t0, "restarts": state.get("restarts", 0)}
+        stop = scan_block(sieve, ck, units_total, unit, chunk, budget, on_launch)
+        if stop:
+            s["wall_accum"] = wall()
+            ck.save()
+            return dict(common(stop), block=t, idx_done=s["idx_done"], unit_done=s["unit_done"],
+                        units_total=units_total, R=plan.R, wall=s["wall_accum"],
+                        checkpoint=ckpt_path)
+        row, good, rejected = conclude_block(plan, k, t, s["survivor_idx"])
+        s["block_audit"].append(row)
+        if rejected or good:
+            ck.save()
+            if rejected:
+                return dict(common("DISAGREEMENT"), block=t,
+                            kernel_survivors=[str(n) for n in sorted(
+                                plan.idx_to_n(i, t) for i in s["survivor_idx"])][:20],
+                            ref_rejected=[str(n) for n in rejected[:20]])
+            return dict(common("FOUND"), g=str(good[0]), block=t, engine=engine,
+                        blocks_scanned=t + 1, survivors_in_block=len(s["survivor_idx"]),
+                        all_good_in_block=[str(n) for n in good])
+        log(f"k={k} block {t}: every position scanned, no survivor passes the referee")
+        ck.next_block()
+        ck.save()
+    return dict(common("EXHAUSTED_BLOCKS"), blocks=s["t"])
 
 
 def main(argv=None) -> None:
-    """argv of harness/run_k_masks.py:230-248; ES_ENGINE selects the engine module."""
+    """CLI with the flags of harness/run_k_masks.py:253-292 (one JSON result line)."""
     signal.signal(signal.SIGTERM, request_stop)
     signal.signal(signal.SIGINT, request_stop)
-    ap = argparse.ArgumentParser(description="resumable block driver for the MI355X engine")
+    ap = argparse.ArgumentParser(description="resumable block driver for an engine module")
     ap.add_argument("k", type=int)
+    ap.add_argument("--engine", default=os.environ.get("ES_ENGINE", "gpu_sieve_mi355x"))
     ap.add_argument("--tag", default="primary")
     ap.add_argument("--ckpt-dir", default=None)
     ap.add_argument("--tune", type=json.loads, default=None)
     ap.add_argument("--wheel", type=json.loads, default=None)
-    ap.add_argument("--ban", default=None, help="comma-separated primes banned from the wheel")
+    ap.add_argument("--ban", default=None, help="comma-separated primes kept out of the wheel")
     ap.add_argument("--max-blocks", type=int, default=None)
     ap.add_argument("--start-block", type=int, default=None)
     ap.add_argument("--max-launches", type=int, default=None)
@@ -235,34 +302,33 @@ def main(argv=None) -> None:
     ap.add_argument("--ckpt-every", type=float, default=120.0)
     ap.add_argument("--min-free-mib", type=int, default=2048)
     ap.add_argument("--no-resume", action="store_true")
-    ap.add_argument("--bfile", default=os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt"))
+    ap.add_argument("--bfile", default=es_data.BFILE)
     ap.add_argument("--known", default=None)
     a = ap.parse_args(argv)
-    mod = importlib.import_module(os.environ.get("ES_ENGINE", "gpu_sieve_mi355x"))
+    eng = importlib.import_module(a.engine)
+
     wheel = a.wheel
     if a.ban:
-        ban = [int(x) for x in a.ban.split(",")]
-        ps = [p for p in es_plan.primes_upto(a.k) if p not in ban]
-        full = mod.Plan(a.k, a.tune)
-        w = mod.plan_wheel(a.k, ps, -math.log2(full.density_all), dict(mod.TUNABLES, **(a.tune or {})))
-        wheel = [[p, T] for p, T in w]
-    ckpt_dir = a.ckpt_dir or os.path.join(ROOT, "checkpoints")
+        banned = {int(x) for x in a.ban.split(",")}
+        base = eng.Plan(a.k, a.tune)
+        keep = [p for p in es_plan.primes_upto(a.k) if p not in banned]
+        wheel = [[p, T] for p, T in eng.plan_wheel(a.k, keep, -math.log2(base.density_all),
+                                                   dict(eng.TUNABLES, **(a.tune or {})))]
+        print(f"# wheel without {sorted(banned)}: {wheel}", file=sys.stderr, flush=True)
+    ckpt_dir = a.ckpt_dir or os.path.join(os.getcwd(), "checkpoints")
     os.makedirs(ckpt_dir, exist_ok=True)
     ckpt = os.path.join(ckpt_dir, f"k{a.k}_{a.tag}.json")
-    res = run(mod, a.k, ckpt, a.tune, wheel, a.max_blocks, a.timeout, a.log_every, a.ckpt_every,
+    res = run(eng, a.k, ckpt, a.tune, wheel, a.max_blocks, a.timeout, a.log_every, a.ckpt_every,
               a.min_free_mib, not a.no_resume, a.start_block, a.max_launches)
-    res["plan"] = mod.Plan(a.k, a.tune, wheel).summary()
-    res["engine_module"] = mod.__name__
-    res["tag"] = a.tag
+    res.update(plan=eng.Plan(a.k, a.tune, wheel).summary(), engine_module=eng.__name__,
+               engine_file=os.path.relpath(eng.__file__, ROOT), tag=a.tag,
+               engine_revision=engine_revision(eng))
     if a.ban:
         res["banned"] = a.ban
-    known = mod.load_known(a.bfile, a.known)
+    known = eng.load_known(a.bfile, a.known)
     if res.get("status") == "FOUND" and a.k in known:
-        g_known, src = known[a.k]
-        res["matches_bfile"] = int(res["g"]) == g_known
-        res["bfile"] = str(g_known)
-        res["known_source"] = src
-    res["engine_revision"] = engine_revision(mod)
+        g, src = known[a.k]
+        res.update(matches_bfile=int(res["g"]) == g, bfile=str(g), known_source=src)
     print(json.dumps(res, default=str), flush=True)
     if res.get("status") in ("DISAGREEMENT", "SKIPPED_LOW_VRAM"):
         raise SystemExit(2)

@@ -187,6 +187,14 @@ class FilterPass:
         nat.check(nat.lib().es_fpass_last_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def read_stream_raw(self, offset: int, count: int):
+        """Stream entries [offset, offset+count) as numpy arrays: values as (lo, hi) u64 pairs
+        ([2 * count]) and their candidate IDs."""
+        v = np.zeros(2 * max(1, count), dtype=np.uint64)
+        ids = np.zeros(max(1, count), dtype=np.uint32)
+        nat.check(nat.lib().es_fpass_read_stream(self._h, offset, count, nat.ptr64(v), nat.ptr32(ids)))
+        return v[:2 * count], ids[:count]
+
     def read_stream(self, offset: int, count: int):
         """(values as python ints, ids) of stream entries [offset, offset+count)."""
         v = np.zeros(2 * max(1, count), dtype=np.uint64)

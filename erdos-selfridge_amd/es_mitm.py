@@ -21,6 +21,8 @@ import math
 
 import numpy as np
 
+import es_data
+
 import es_native as nat
 import es_plan as EP
 
@@ -264,7 +266,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     bf = {}
     here = os.path.dirname(os.path.abspath(__file__))
-    path = a.bfile or os.path.join(os.path.dirname(here), "tests", "golden", "b003458_1to400.txt")
+    path = a.bfile or es_data.BFILE
     if os.path.exists(path):
         for line in open(path):
             t = line.split()

@@ -38,6 +38,8 @@ TUNABLES = {
     "cost_scale": 256,         # DP cells per bit of log2(M)
     "m_cap_bits": 120,         # hard cap on log2(M)
     "max_R_bits": 62,          # residues per block must index in uint64
+    "target_g": 0,             # > 0: a KNOWN g (confirmation runs): the wheel minimises the
+                               # positions scanned to cover [0, g), ceil(g / M) * R
     "threads": "auto",         # workgroup size ("auto": the engine picks the size of 256,
                                # 512, 1024 that keeps the most waves resident per CU)
     "chunk_log2": 30,          # positions per launch unit of the reference (drivers read ochunk)
@@ -170,7 +172,8 @@ def plan_wheel(k: int, ps: list[int], log2E: float, tun: dict) -> list[tuple[int
     """Choose the wheel items for k from candidate primes ps (sieve/gpu_sieve_v9.py:187-227).
 
     truecost: minimise log2(R) - log2(1 - exp(-M/E)) over the DP cells (expected checks
-    including block overshoot); knapsack: maximise log2(M) - log2(R) under log2(n_mult*E).
+    including block overshoot); knapsack: maximise log2(M) - log2(R) under log2(n_mult*E);
+    target_g > 0 (g already known, confirmation runs): minimise ceil(g/M) * R.
     The exact integers M < 2^126 and R < 2^62 are then enforced by dropping top digits."""
     scale = int(tun.get("cost_scale", 256))
     max_M_bits = min(int(tun.get("m_cap_bits", 120)), MAX_M_BITS)
@@ -187,6 +190,17 @@ def plan_wheel(k: int, ps: list[int], log2E: float, tun: dict) -> list[tuple[int
         score = np.where(ok, cidx / scale - best, -np.inf)
         c = int(np.argmax(score))
         if not ok[c]:
+            c = 0
+    elif int(tun.get("target_g", 0)) > 0:
+        # g known (sieve/gpu_sieve_v11.py:199-213): covering [0, g) takes ceil(g / M) blocks
+        # of R positions, so the cost of cell c is log2 R + log2 ceil(2^(log2 g - c/scale))
+        over = math.log2(int(tun["target_g"])) - cidx / scale
+        with np.errstate(over="ignore", invalid="ignore"):
+            blocks = np.where(over <= 0.0, 1.0, np.ceil(np.exp2(np.clip(over, -200.0, 200.0))))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            cost = np.where(ok & np.isfinite(blocks), best + np.log2(blocks), np.inf)
+        c = int(np.argmin(cost))
+        if not np.isfinite(cost[c]):
             c = 0
     else:
         x = np.minimum(2.0 ** np.clip(cidx / scale - log2E, -200.0, 60.0), 1e300)

@@ -24,6 +24,7 @@ import time
 
 import numpy as np
 
+import es_data
 import es_native as nat
 from es_plan import (MAX_M_BITS, TUNABLES, Plan, allowed_residues, base_digits, is_good,  # noqa: F401
                      order_filters, partial_count, partial_residues, plan_wheel, primes_upto)
@@ -630,7 +631,7 @@ def main(argv=None):
     ap.add_argument("--max-blocks", type=int, default=None)
     ap.add_argument("--timeout", type=float, default=None, help="seconds; process exits 124 on expiry")
     ap.add_argument("--min-free-mib", type=int, default=2048)
-    ap.add_argument("--bfile", default=os.path.join(ROOT, "tests", "golden", "b003458_1to400.txt"))
+    ap.add_argument("--bfile", default=es_data.BFILE)
     ap.add_argument("--known", default=None, help="extra 'k g' file of known terms")
     a = ap.parse_args(argv)
     if a.timeout:

@@ -296,6 +296,41 @@ def c_filter_values(k: int, primes, values, ids, cap: int = 1 << 22):
     return out[:min(n, cap)].copy()
 
 
+def c_filter_lohi(k: int, primes, lohi, ids, threads: int = 8, cap: int = 1 << 24):
+    """c_filter_values on a stream held as (lo, hi) u64 pairs (numpy), the IDs split over
+    `threads` concurrent calls (ctypes releases the GIL).  Returns the survivor IDs sorted."""
+    import threading
+
+    import numpy as np
+    L = lib()
+    lohi = np.ascontiguousarray(lohi, dtype=np.uint64)
+    ids = np.ascontiguousarray(np.asarray(ids, dtype=np.uint32))
+    nv = lohi.size // 2
+    u32p, u64p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+    pr = _u32(list(primes))
+    parts = np.array_split(ids, max(1, threads))
+    outs = [None] * len(parts)
+    errs = []
+
+    def work(i):
+        part = np.ascontiguousarray(parts[i])
+        out = np.zeros(max(1, cap), dtype=np.uint32)
+        n = L.eso_filter_values(k, len(primes), pr, lohi.ctypes.data_as(u64p), nv,
+                                part.ctypes.data_as(u32p), part.size, out.ctypes.data_as(u32p),
+                                cap)
+        if n < 0 or n > cap:
+            errs.append(n)
+        outs[i] = out[:max(0, min(n, cap))]
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(parts))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    if errs:
+        raise ValueError(f"oracle rejected the input or overflowed: {errs}")
+    return np.sort(np.concatenate(outs)) if outs else np.zeros(0, dtype=np.uint32)
+
+
 def c_good_range(k: int, n_lo: int, n_hi: int, threads: int = 8, cap: int = 1 << 20):
     """(count, ascending list) of every n in [n_lo, n_hi) dominating k for all p <= k
     (eso_good_range: brute force, independent of any wheel)."""

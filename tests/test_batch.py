@@ -104,3 +104,37 @@ def test_graph_sweep_matches_eager_every_replay():
     # the eager batch run agrees with the last replay
     b.run()
     assert [b.collect(jid) for jid, _, _, _ in jobs] == got
+
+
+def test_config3_full_size_autotuned():
+    """Config 3 at its size: k = 300..320 with autotuned launch shapes, 4 seeded blocks x 2^34
+    positions per k in one batch; every job equals the unbatched engine, and a seeded 2^28
+    sub-window of every job equals the oracle (full filter set)."""
+    import es_autotune
+    import es_batch
+    import es_oracle as O
+    rng = random.Random(300320)
+    N = 1 << 34
+    space = dict(es_autotune.SPACE, mi_global_masks=[0])     # batches read masks from LDS
+    b = es_batch.Batch()
+    jobs = []
+    for k in range(300, 321):
+        tuned, _ = es_autotune.autotune(k, space=space, probe_log2=30, method="coord")
+        eng = _engine(k, **tuned)
+        print(f"case config3 k={k} tuned={tuned}", flush=True)
+        for _ in range(4):
+            t = rng.randrange(1 << 20)
+            lo = rng.randrange(eng.plan.R - N)
+            jobs.append((b.add(eng, t, lo, lo + N), eng, t, lo, lo + N))
+    print(f"case config3 batch jobs={len(jobs)}", flush=True)
+    ms, launches = b.run()
+    print(f"  batch {ms:.1f} ms, {launches} launches", flush=True)
+    for jid, eng, t, lo, hi in jobs:
+        got = b.collect(jid)
+        want, _ = eng.scan_window(t, lo, hi)
+        assert got == sorted(want), (eng.plan.k, t, lo)
+        sub = lo + rng.randrange(N - (1 << 28))
+        plan = eng.plan
+        print(f"  job k={plan.k} t={t} oracle window [{sub}, +2^28)", flush=True)
+        n, ids = O.c_scan(plan.k, plan.wheel, plan.rest, t, sub, sub + (1 << 28), threads=16)
+        assert [i for i in got if sub <= i < sub + (1 << 28)] == [int(x) for x in ids]

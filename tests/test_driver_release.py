@@ -117,3 +117,48 @@ def test_duplicate_bfile_index_is_rejected(tmp_path):
     path.write_text("1 3\n1 5\n")
     with pytest.raises(ValueError, match="missing, repeated, or unordered"):
         es_release.read_table(path, 2)
+
+
+def test_timeout_saves_and_reports(tmp_path):
+    res = _run(tmp_path, timeout=1e-9)
+    assert res["status"] == "TIMEOUT" and os.path.exists(tmp_path / "k30.json")
+    again = _run(tmp_path)
+    assert again["status"] == "FOUND" and again["restarts"] == 1
+
+
+def test_corrupt_checkpoint_is_refused(tmp_path):
+    _run(tmp_path, max_launches=2)
+    ck = json.load(open(tmp_path / "k30.json"))
+    ck["survivor_idx"] = ck["survivor_idx"] + [10 ** 9]          # beyond the scanned prefix
+    json.dump(ck, open(tmp_path / "k30.json", "w"))
+    with pytest.raises(AssertionError, match="scanned prefix"):
+        _run(tmp_path)
+
+
+def test_source_revision_checked_against_a_tree(tmp_path, term397):
+    """engine_revision hashes are verified only against a given source tree."""
+    run = deepcopy(term397["primary"])
+    src = tmp_path / "src"
+    (src / "sieve").mkdir(parents=True)
+    (src / "sieve" / "engine.py").write_text("kernel\n")
+    import hashlib
+    good = hashlib.sha256(b"kernel\n").hexdigest()
+    run["engine_revision"] = {"sieve/engine.py": good}
+    n = int(term397["n"])
+    assert es_release.check_run(397, n, run, source_root=src) == len(run["block_audit"])
+    run["engine_revision"] = {"sieve/engine.py": "0" * 64}
+    assert es_release.check_run(397, n, run) == len(run["block_audit"])   # no tree: not checked
+    with pytest.raises(ValueError, match="source revision mismatch"):
+        es_release.check_run(397, n, run, source_root=src)
+
+
+def test_wheel_facts_closed_form():
+    """Residues per block = prod over wheel items of prod_{i<T} (p - a_i)
+    (harness/verify_release.py:56-59), and an exponent beyond k's digit count is invalid."""
+    f = es_release.WheelFacts.derive(30, [[2, 5], [3, 4], [5, 3], [7, 2], [11, 2], [13, 2]])
+    assert f.modulus == 2**5 * 3**4 * 5**3 * 7**2 * 11**2 * 13**2
+    assert f.residues == 230947200
+    with pytest.raises(ValueError, match="invalid wheel"):
+        es_release.WheelFacts.derive(30, [[2, 6]])
+    with pytest.raises(ValueError, match="repeated wheel prime"):
+        es_release.WheelFacts.derive(30, [[2, 1], [2, 2]])

@@ -160,3 +160,21 @@ def test_gpu_pass_shapes_agree_ragged():
         for n_first in (0, 1, 2, 5, 8):
             fp.set_shape(unroll, n_first)
             assert np.array_equal(fp.run(), want), (unroll, n_first)
+
+
+@pytest.mark.gpu
+def test_gpu_pass_config4_full_size():
+    """Config 4 at its full size: 2^30 seeded candidates of the conditioned u128 stream, all 53
+    filters in the conditional order the bench uses; every survivor ID equals the oracle's
+    Kummer test of every stream value (16 host threads)."""
+    n = 1 << 30
+    spec = FP.generator_spec(K, FP.filter_order(K, PRIMES))
+    fp = FP.FilterPass(K, FP.conditional_order(K, PRIMES, spec), order="given")
+    print(f"case fpass full size n=2^30 filters={len(fp.primes)}", flush=True)
+    fp.generate(n, seed=20260901, spec=spec)
+    got = fp.run()
+    lohi, ids = fp.read_stream_raw(0, n)
+    assert ids.size == n
+    want = O.c_filter_lohi(K, fp.primes, lohi, ids, threads=16)
+    print(f"  survivors={len(got)} oracle={len(want)}", flush=True)
+    assert len(want) > 100 and np.array_equal(got, want)

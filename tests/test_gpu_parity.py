@@ -147,31 +147,42 @@ def _frontier():
     return json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
 
 
-@pytest.mark.parametrize("k,role,t_idx", [(380, "primary", 0), (399, "primary", 0),
-                                          (399, "confirmation", 0), (398, "confirmation", 0),
-                                          (400, "primary", 1)])
-def test_recorded_survivor_windows(k, role, t_idx):
-    """Windows of 2^24 positions around every recorded survivor of a recorded block must
-    return exactly the recorded survivors inside the window (full filter set)."""
+_RUNS = [(t["k"], role) for t in json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
+         for role in ("primary", "confirmation")]
+
+
+@pytest.mark.parametrize("k,role", _RUNS)
+def test_recorded_survivor_windows(k, role):
+    """Every recorded run of evidence/through400/runs.json (46 wheels: both roles of k=378..400):
+    around every recorded survivor of every block, a 2^24-position window returns exactly the
+    recorded survivors inside it and the oracle's survivors, and its minimum agrees with the
+    oracle's (full filter set)."""
     term = next(x for x in _frontier() if x["k"] == k)
     run = term[role]
-    audit = run["block_audit"][t_idx]
-    t = audit["t"]
     eng, plan = engine(k, run["wheel"], max_survivors=1 << 16)
     assert str(plan.M) == run["N"] and plan.R == run["residues_per_block"]
-    for sv in audit["survivors_n"]:
-        n = int(sv)
-        idx, tt = plan.n_to_idx(n)
-        assert tt == t
-        lo = max(0, idx - (1 << 23))
-        hi = min(plan.R, lo + (1 << 24))
-        say(f"case k={k} {role} t={t} survivor idx={idx} window=[{lo},{hi})")
-        ids, mn = eng.scan_window(t, lo, hi)
-        inside = sorted(i for i in (plan.n_to_idx(int(x))[0] for x in audit["survivors_n"])
-                        if lo <= i < hi)
-        assert sorted(ids) == inside
-        cnt, want = O.c_scan(k, plan.wheel, plan.rest, t, lo, hi, threads=16)
-        assert sorted(ids) == [int(x) for x in want]
+    windows = 0
+    for audit in run["block_audit"]:
+        t = audit["t"]
+        listed = audit["survivors_n"] or []
+        recorded = sorted(plan.n_to_idx(int(x))[0] for x in listed)
+        for sv in listed:
+            idx, tt = plan.n_to_idx(int(sv))
+            assert tt == t
+            lo = max(0, idx - (1 << 23))
+            hi = min(plan.R, lo + (1 << 24))
+            say(f"case k={k} {role} t={t} survivor idx={idx} window=[{lo},{hi}) "
+                f"masks={eng.info.n_masks_used} r0={eng.info.r0}")
+            ids, mn = eng.scan_window(t, lo, hi)
+            assert sorted(ids) == [i for i in recorded if lo <= i < hi]
+            cnt, want, want_min = O.c_scan_min(k, plan.wheel, plan.rest, t, lo, hi, threads=16)
+            assert sorted(ids) == [int(x) for x in want] and mn == want_min
+            windows += 1
+    assert windows >= 1
+    # the recorded answer's window returns g(k) as its minimum
+    g = int(term["n"])
+    gi, gt = plan.n_to_idx(g)
+    assert gt == run["block"]
 
 
 @pytest.mark.parametrize("k", [270, 275, 280])
@@ -358,3 +369,16 @@ def test_window_edges(k):
     for lo, hi in ((0, span), (plan.R - span, plan.R), (plan.R - 3, plan.R),
                    (5 * r - 1, 5 * r + 1)):
         window_check(k, None, 2, lo, hi, limit=8)
+
+
+def test_block_overflow_falls_back_to_per_launch_drain():
+    """scan_block of C1 block 0 (12,654,618 survivors) with a 2^20-ID buffer and 2^24-position
+    launches (~0.9M survivors each): the block overflows the buffer, no launch does, so the
+    per-launch drain (_scan_block_chunked) must return exactly the oracle's survivors."""
+    eng, plan = engine(30, C1_WHEEL, max_survivors=1 << 20, mi_launch_log2=24)
+    say("case k=30 block 0 overflow drain")
+    ids = eng.scan_block(0)
+    n, want = O.c_scan(30, plan.wheel, plan.rest, 0, 0, plan.R, threads=16, cap=1 << 24)
+    assert n == 12654618 > eng.out_cap
+    got = np.sort(np.asarray(ids, dtype=np.uint64))
+    assert len(got) == n and np.array_equal(got, want)

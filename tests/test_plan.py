@@ -47,7 +47,7 @@ def test_recorded_frontier_plans():
     checked = set()
     for x in plans:
         if x["tag"] not in ("primary", "confirm") or x["log"].endswith("k392_confirm.log"):
-            continue                      # other planners (target_g, v10) or manual wheels
+            continue                      # target_g plans (next test) or an earlier planner
         k, rec = x["k"], x["plan"]
         key = (k, json.dumps(rec["wheel"]))
         if key in checked:
@@ -228,3 +228,41 @@ def test_lead_block_rule():
     assert G.swap_lead([108, 26, 20]) == 1           # four words
     assert G.swap_lead([40, 30], inner_mod=3 ** 6) == 0 and G.swap_lead([30, 20]) == 0
     assert G.swap_lead([100, 90, 90, 20]) == 0       # lead block above the cap
+
+
+def test_target_g_plans_reproduced():
+    """The known-g planner (sieve/gpu_sieve_v11.py:199-213): the recorded k=384 v11 gate,
+    k=388 v10 check and k=392 confirmation plans bit for bit (float summary fields included),
+    and the runs.json confirmation wheels it produced."""
+    import es_data
+    b = es_data.known_terms()
+    plans = json.load(open(os.path.join(GOLDEN, "frontier_plans.json")))
+    seen = 0
+    for x in plans:
+        if x["tag"] in ("v10check", "v11gate") or x["log"].endswith("k392_confirm.log"):
+            k, rec = x["k"], x["plan"]
+            s = EP.Plan(k, {"target_g": b[k]}).summary()
+            for f in ("wheel", "M", "residues_per_block", "n_filters", "density_wheel",
+                      "density_rest", "expected_g"):
+                assert s[f] == rec[f], (k, x["tag"], f)
+            seen += 1
+    assert seen == 5
+    terms = {t["k"]: t for t in json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]}
+    for k in (385, 387, 389, 390, 392, 395, 396, 400):
+        plan = EP.Plan(k, {"target_g": b[k]})
+        assert [[p, T] for p, T in plan.wheel] == terms[k]["confirmation"]["wheel"], k
+        assert b[k] // plan.M == terms[k]["confirmation"]["block"]
+    # the branch is live: a known g changes the default plan
+    assert EP.Plan(385).wheel != EP.Plan(385, {"target_g": b[385]}).wheel
+
+
+def test_target_g_cost_is_blocks_times_R():
+    """Among the planner's candidate cells the chosen wheel covers [0, g) with the fewest
+    positions: no other wheel of the same DP (same cells) with a smaller ceil(g/M) * R."""
+    import es_data
+    k = 300
+    g = es_data.known_terms()[k]
+    plan = EP.Plan(k, {"target_g": g})
+    cost = -(-g // plan.M) * plan.R
+    default = EP.Plan(k)
+    assert cost <= -(-g // default.M) * default.R

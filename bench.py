@@ -13,6 +13,7 @@ the host cores on a bounded sub-window of the same block.
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -44,6 +45,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-part2", action="store_true",
+                    help="sieve: leave out the surrogate-vs-plain time to first nonempty block")
     ap.add_argument("--autotune", action="store_true",
                     help="batch, k200: autotune the launch shape of each wheel first")
     ap.add_argument("--workload", choices=("sieve", "fpass", "copy", "stream", "valu", "lds",
@@ -149,6 +152,7 @@ def main() -> int:
     lds_peak = lds_read_peak()
     roofline = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
     cpu = None if a.no_cpu else cpu_baseline(plan, eng, g_idx, a)
+    part2 = None if a.no_part2 else first_block_ab()
 
     print(json.dumps({
         "metric": "residue positions sieved/s (k=399 wheel); time to first nonempty block vs plain",
@@ -159,9 +163,67 @@ def main() -> int:
                                f"2^{a.window_log2} positions per step, all 75 filters",
                    "k": K, "r0": eng.r0, "R": plan.R, "window": N, "masks": F,
                    "strip": eng.strip_len, "threads": eng.threads},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "time_to_first_block": part2,
     }), flush=True)
     return 0
+
+
+def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 313, 317),
+                   pairs: int = 2) -> dict:
+    """Metric part 2 measured end to end: time to exhaust the first nonempty block with the
+    surrogate-planned wheel against the plain planner's wheel, on k whose first nonempty block
+    scans in about a second (the frontier k = 350..400 take hours per block: bench.py
+    --workload surrogate estimates those from measured rates).  The network is trained on
+    wheels the sieve probes at even k = 300..320; per odd k (not trained on) it ranks 10^4
+    candidate wheels, the first 8 and the plain wheel are probed, and the least E_pos x ns
+    wins.  Both wheels then run the full search (GpuSieve.search: blocks t = 0, 1, ... until
+    the referee accepts a survivor) in interleaved pairs (plain, surrogate, surrogate, plain,
+    ...; harness/paired_bench.py:22-68), each must return the b-file g(k), and each wheel's
+    time is its median over the pairs.  Engine construction is outside the timed search."""
+    import es_data
+    import es_plan
+    import es_surrogate as S
+    import gpu_sieve_mi355x as G
+    import numpy as np
+    bf = es_data.known_terms()
+    t0 = time.perf_counter()
+    X, Y, _ = S.training_set(list(train_ks), per_k=24, n_candidates=10000, seed=5, probe_log2=33)
+    model = S.Surrogate(seed=3)
+    model.fit(X, Y, epochs=300, batch=32)
+    t_train = time.perf_counter() - t0
+    rows = []
+    for k in eval_ks:
+        res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=8, probe_log2=33, seed=k)
+        wheels = {"plain": res["plain"]["wheel"], "surrogate": res["best"]["wheel"]}
+        engines = {tag: G.GpuSieve(es_plan.Plan(k, None, w)) for tag, w in wheels.items()}
+        times = {"plain": [], "surrogate": []}
+        found = {}
+        for i in range(pairs):
+            order = ("plain", "surrogate") if i % 2 == 0 else ("surrogate", "plain")
+            for tag in order:
+                w0 = time.perf_counter()
+                r = engines[tag].search(max_blocks=64)
+                times[tag].append(time.perf_counter() - w0)
+                assert r["status"] == "FOUND" and r["g"] == bf[k], (k, tag, r.get("status"))
+                found[tag] = (r["g"], r["block"])
+        med = {tag: float(np.median(v)) for tag, v in times.items()}
+        rows.append({"k": k, "plain_s": med["plain"], "surrogate_s": med["surrogate"],
+                     "ratio": med["surrogate"] / med["plain"],
+                     "same_wheel": wheels["plain"] == wheels["surrogate"],
+                     "blocks": {tag: found[tag][1] + 1 for tag in found},
+                     "R": {tag: engines[tag].plan.R for tag in engines},
+                     "expected_ratio": res["best"]["cost_s"] / res["plain"]["cost_s"]})
+    ps = sum(r["plain_s"] for r in rows)
+    ss = sum(r["surrogate_s"] for r in rows)
+    return {"metric": "time to first nonempty block, surrogate-planned / plain-planned wheel",
+            "value": ss / ps, "unit": "ratio (sum of median search times over k)",
+            "higher_is_better": False,
+            "geomean_ratio": float(np.exp(np.mean([math.log(r["ratio"]) for r in rows]))),
+            "config": {"workload": f"searches of k={','.join(map(str, eval_ks))} from block 0 "
+                                   f"to g(k); network trained on k={train_ks[0]}..{train_ks[-1]}"
+                                   f" even, {len(X)} probed wheels", "pairs": pairs,
+                       "candidates": 10000, "shortlist": 8},
+            "train_and_plan_s": t_train, "per_k": rows}
 
 
 # LDS read peak of gfx950 at the nominal clock: 256 CUs x 256 bytes per CU-clock (64 banks of

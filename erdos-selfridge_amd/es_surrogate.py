@@ -476,5 +476,6 @@ def surrogate_plan(k: int, model: Surrogate | None, n_candidates: int = 100_000,
     if log:
         log(f"k={k}: {len(E)} candidates ranked in {t_rank:.1f} s; best {best['cost_s']:.1f} s "
             f"(plain {plain_r['cost_s']:.1f} s)")
-    return {"k": k, "best": best, "plain": plain_r, "probed": [v for v in cache.values() if v],
+    return {"k": k, "best": best, "plain": plain_r, "first_pick": probed(rank[0]),
+            "probed": [v for v in cache.values() if v],
             "n_candidates": len(E), "rank_seconds": t_rank, "sizes": sizes}

@@ -1,7 +1,6 @@
 """Learned planner: vectorised wheel arithmetic against Plan and the engine's kernel geometry
-(CPU); on the GPU, a surrogate trained on wheels the sieve probes itself picks wheels faster
-than the plain planner's where the plain wheel is slow on this kernel, with identical exact
-minima."""
+(CPU); on the GPU, what a surrogate trained on wheels the sieve probes itself gains over the
+analytic-only ranking at a fixed probe budget, with identical exact minima."""
 from __future__ import annotations
 
 import math
@@ -64,46 +63,53 @@ def test_plain_vector_roundtrip():
     assert sorted(sp.wheel(sp.vector(w))) == sorted(w)
 
 
-def _spearman(a, b):
-    ra = np.argsort(np.argsort(a))
-    rb = np.argsort(np.argsort(b))
-    return float(np.corrcoef(ra, rb)[0, 1])
-
-
 @pytest.mark.gpu
-def test_surrogate_ranks_unseen_wheels_and_keeps_minima():
-    """Trained on 512 wheels the sieve probes at even k = 370..400, the network ensemble ranks
-    64 unseen wheels of k = 397 (Spearman correlation of predicted and measured log2 ns and
-    log2 density, printed and bounded below; measured 0.42-0.48 and 0.92-0.96 over training
-    seeds on this kernel, whose speeds differ less between wheels than earlier kernels' did --
-    with 256 training and 32 unseen wheels the ns correlation came out 0.30-0.47, too close to
-    the bound); the probed shortlist is never slower than the plain wheel; the chosen and
-    plain wheels return the same exact minimum g(k)."""
-    import gpu_sieve_mi355x as G
-    X, Y, meta = S.training_set(list(range(370, 401, 2)), per_k=32, n_candidates=10000, seed=2)
+def test_network_beats_analytic_shortlist_at_fixed_probe_budget():
+    """What the trained network gains over the method without it.  Trained on 512 wheels the
+    sieve probes at even k = 370..400, the planner ranks 10^4 candidate wheels of eight unseen
+    odd k = 371..399 by analytic E_pos + predicted ns; the analytic-only ranking (E_pos alone)
+    is the method without the network.  From one shared pool of probes (measured ns per
+    position), per k and shortlist size s: best cost E_pos x ns of each ranking's first s
+    wheels plus the plain wheel, over the plain wheel's cost.  Geomeans over the eight k,
+    measured on this kernel over three network seeds: the network's first pick with no probe
+    0.91-0.93 (the analytic first pick is the plain wheel: 1.00); s = 2: 0.88-0.90 against
+    0.92.  Asserted with margin: first pick <= 0.97, and the network's s = 1 and s = 2 gains
+    over the analytic ranking sum to >= 0.05 with s = 2 no worse.  The chosen and plain wheels
+    return the same exact minimum g(k)."""
+    X, Y, meta = S.training_set(list(range(370, 401, 2)), per_k=32, n_candidates=10000, seed=2,
+                                probe_log2=33)
     assert len(X) >= 400
     model = S.Surrogate(seed=2)
     loss, _ = model.fit(X, Y, epochs=300, batch=32)
     assert np.isfinite(loss)
-    Xh, Yh, _ = S.training_set([397], per_k=64, n_candidates=10000, seed=11)
-    rho = _spearman(model.predict(Xh)[:, 1], Yh[:, 1])
-    rho_d = _spearman(model.predict(Xh)[:, 0], Yh[:, 0])
-    print(f"unseen k=397: Spearman(pred, measured) log2 ns {rho:.3f}, log2 density {rho_d:.3f}",
-          flush=True)
-    assert rho > 0.3 and rho_d > 0.7
-    gs = bfile()
-    for k in (391, 400):
-        res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=4, probe_log2=32, seed=2,
-                               compare=(1, 4))
+    rows = []
+    for k in range(371, 400, 4):
+        res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=1, probe_log2=33, seed=k,
+                               compare=(1, 2))
         print(f"k={k} sizes {res['sizes']}", flush=True)
-        assert res["best"]["cost_s"] <= res["plain"]["cost_s"]
-        for wheel in (res["best"]["wheel"], res["plain"]["wheel"]):
+        rows.append(res)
+
+    def gm(key, m):
+        return float(np.exp(np.mean([math.log(r["sizes"][key][m]) for r in rows])))
+    top1 = gm("top1_no_probe", "surrogate")
+    s1_s, s1_a = gm(1, "surrogate"), gm(1, "analytic")
+    s2_s, s2_a = gm(2, "surrogate"), gm(2, "analytic")
+    print(f"geomean cost / plain: first pick {top1:.3f}; s=1 {s1_s:.3f} vs {s1_a:.3f}; "
+          f"s=2 {s2_s:.3f} vs {s2_a:.3f}", flush=True)
+    assert s1_a == 1.0                       # the analytic first pick is the plain wheel
+    assert top1 <= 0.97
+    assert s2_s <= s2_a and (s1_a - s1_s) + (s2_a - s2_s) >= 0.05
+    gs = bfile()
+    for k in (391, 399):
+        res = next(r for r in rows if r["k"] == k)
+        first = res["first_pick"] or res["plain"]
+        for wheel in (first["wheel"], res["plain"]["wheel"]):
             plan = EP.Plan(k, None, wheel)
             eng = G.GpuSieve(plan)
             gi, gt = plan.n_to_idx(gs[k])
             lo = max(0, gi - (1 << 30))
             _, mn = eng.scan_window(gt, lo, min(plan.R, lo + (1 << 31)))
-            assert mn == gs[k]
+            assert mn == gs[k], (k, wheel)
 
 
 def test_features_follow_digit_ring_split():

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "es_common.h"
@@ -1206,3 +1207,128 @@ int es_batch_collect(es_batch* b, uint32_t job, uint64_t* ids, uint64_t cap, uin
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ window scan without allocation
+//
+// es_engine_scan_window_static: the window scan and collect of an engine for callers that
+// cannot allocate device memory (HIP graph capture, serving loops).  The kernels' scratch --
+// the survivor counter, the slot of the least residue and its floor, one residue record per
+// survivor slot, and the exact test's copy of the plan record that points at them -- lives in
+// one file-scope device array, `scratch`; survivor IDs go to the caller's device buffer and
+// the count and minimum to the caller's device result.  Every GPU holds its own copy of the
+// array until the process ends; calls that overlap on one GPU race on it.
+
+namespace {
+
+constexpr uint64_t kStaticMaxIds = 65536;      // survivor slots the scratch records
+constexpr size_t kStaticColdOff = 0;           // ColdPlan of the call (the exact test's view)
+constexpr size_t kStaticCtrOff = 256;          // count, 1 + slot of the minimum, floor
+constexpr size_t kStaticRecOff = 512;          // [kStaticMaxIds][2] residues (lo, hi)
+constexpr size_t kScratchBytes = kStaticRecOff + kStaticMaxIds * 16;
+static_assert(sizeof(ColdPlan) <= kStaticCtrOff - kStaticColdOff, "ColdPlan outgrew its slot");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[64] = {};         // device address of `scratch`, per device
+
+// Device address of this device's scratch array, looked up once per device (the array's
+// name is a host-side stand-in, never a device address).
+int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= 64) return fail(ES_ERR_DEVICE, "device index outside 0..63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    ES_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ES_ERR_DEVICE, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return ES_OK;
+}
+
+// The call's plan record, zeroed counters and the minimum's floor: written by a kernel (a
+// copy or memset node would not be safe to capture).
+__global__ void es_static_setup_kernel(ColdPlan cold, unsigned char* __restrict__ base,
+                                       uint64_t floor) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *reinterpret_cast<ColdPlan*>(base + kStaticColdOff) = cold;
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base + kStaticCtrOff);
+    ctr[0] = 0;
+    ctr[1] = 0;
+    ctr[2] = floor;
+  }
+}
+
+// result = {survivors (may exceed cap), least s lo, least s hi, 1 if there is a minimum}.
+__global__ void es_static_finish_kernel(const unsigned char* __restrict__ base,
+                                        uint64_t* __restrict__ result) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const unsigned long long* ctr =
+        reinterpret_cast<const unsigned long long*>(base + kStaticCtrOff);
+    const unsigned long long* rec =
+        reinterpret_cast<const unsigned long long*>(base + kStaticRecOff);
+    const uint64_t best = ctr[1];
+    result[0] = ctr[0];
+    result[1] = best ? rec[2 * (best - 1)] : 0;
+    result[2] = best ? rec[2 * (best - 1) + 1] : 0;
+    result[3] = best ? 1 : 0;
+  }
+}
+
+}  // namespace
+
+extern "C" int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint64_t idx_hi,
+                                            uint64_t* ids, uint64_t cap, uint64_t* result,
+                                            void* stream) {
+  if (!e || !result || (cap && !ids)) return fail(ES_ERR_INVALID, "null argument");
+  if (idx_hi > (uint64_t)e->R || idx_lo > idx_hi) return fail(ES_ERR_INVALID, "window outside [0, R)");
+  if (cap > kStaticMaxIds)
+    return fail(ES_ERR_LIMIT, "cap exceeds the static scratch: at most 65536 survivor slots");
+  if (!e->t_known)
+    return fail(ES_ERR_INVALID, "the minimum needs the block index: set it with "
+                                "es_engine_set_block (es_engine_set_tconst leaves it unknown)");
+  int dev = -1;
+  ES_HIP(hipGetDevice(&dev));
+  if (dev != e->device) return fail(ES_ERR_INVALID, "the engine's device is not the current one");
+  unsigned char* base = nullptr;
+  int rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
+  ColdPlan cold = e->cold;
+  cold.out = reinterpret_cast<unsigned long long*>(ids);
+  cold.out_n = reinterpret_cast<unsigned long long*>(base + kStaticCtrOff);
+  cold.out_s = reinterpret_cast<unsigned long long*>(base + kStaticRecOff);
+  cold.out_cap = cap;
+  const uint64_t floor = e->t == 0 ? (uint64_t)e->k + 2 : 0;
+  hipLaunchKernelGGL(es_static_setup_kernel, dim3(1), dim3(64), 0, st, cold, base, floor);
+  ES_HIP(hipGetLastError());
+  HotPlan hp = e->hp;
+  hp.cold = reinterpret_cast<const ColdPlan*>(base + kStaticColdOff);
+  if (idx_hi > idx_lo) {
+    uint64_t o_lo, o_hi;
+    outer_range(e, idx_lo, idx_hi, &o_lo, &o_hi);
+    const uint64_t n = (o_hi - o_lo + e->launch_outer - 1) / e->launch_outer;
+    uint64_t per = (o_hi - o_lo + n - 1) / n;
+    per = (per + e->lead - 1) / e->lead * e->lead;
+    for (uint64_t a = o_lo; a < o_hi; a += per) {
+      uint64_t lo = a, hi = std::min(o_hi, a + per);
+      uint64_t u_lo = lo / e->S, u_hi = (hi + e->S - 1) / e->S;
+      uint64_t grid = (u_hi - u_lo + e->threads - 1) / e->threads;
+      grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, e->max_grid));
+      uint64_t wlo = idx_lo, whi = idx_hi;
+      void* args[] = {&hp, &u_lo, &u_hi, &lo, &hi, &wlo, &whi};
+      ES_HIP(hipLaunchKernel((const void*)e->kern, dim3((uint32_t)grid), dim3(e->threads), args,
+                             e->lds_bytes, st));
+      ES_HIP(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(es_static_finish_kernel, dim3(1), dim3(64), 0, st,
+                     (const unsigned char*)base, result);
+  ES_HIP(hipGetLastError());
+  return ES_OK;
+}

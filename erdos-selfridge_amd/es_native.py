@@ -73,6 +73,8 @@ def lib() -> C.CDLL:
         "es_engine_collect": (C.c_int, [eng, u64p, C.c_uint64, u64p, u64p, C.POINTER(C.c_int)]),
         "es_engine_sync": (C.c_int, [eng]),
         "es_engine_kernel_time": (C.c_int, [eng, C.POINTER(C.c_double), u64p, u64p]),
+        "es_engine_scan_window_static": (C.c_int, [eng, C.c_uint64, C.c_uint64, C.c_void_p,
+                                                   C.c_uint64, C.c_void_p, C.c_void_p]),
         "es_batch_create": (C.c_int, [C.c_uint64, C.POINTER(eng)]),
         "es_batch_destroy": (C.c_int, [eng]),
         "es_batch_add": (C.c_int, [eng, eng, C.c_uint64, C.c_uint64, C.c_uint64, u32p]),

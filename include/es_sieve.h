@@ -154,6 +154,20 @@ int es_engine_sync(es_engine* e);
  * stream), their launch count, and the positions they covered. */
 int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t* positions);
 
+/* Window scan without allocation: scans positions [idx_lo, idx_hi) of the engine's current
+ * block (es_engine_set_block) like es_engine_scan_window and writes, on `stream` (NULL: the
+ * engine's stream), the survivor IDs to the caller's device buffer ids[0, min(count, cap)) and
+ * result[4] = {count (may exceed cap), least residue s lo, hi (n = tM + s > k+1), 1 if there
+ * is a minimum else 0} to the caller's device memory.  Asynchronous; allocates nothing: its
+ * scratch (counters, the minimum's slot, one 16-byte residue record per survivor slot, the
+ * exact test's plan record) is a file-scope device array of 1 MiB + 512 bytes that every GPU
+ * holds until the process ends.  cap above ES_STATIC_MAX_IDS fails with ES_ERR_LIMIT.  Calls
+ * that overlap on one GPU race on the shared scratch.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (HIP graph capture, tight serving loops). */
+#define ES_STATIC_MAX_IDS 65536u
+int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t* ids,
+                                 uint64_t cap, uint64_t* result, void* stream);
+
 /* ---- Batched launches over many plans and blocks ---------------------------------------
  * A batch holds jobs (engine, block t, window [idx_lo, idx_hi)); es_batch_run sieves all of
  * them with one launch per kernel shape (mask count, words per ring-0 row, workgroup size):

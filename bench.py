@@ -42,6 +42,8 @@ def main() -> int:
     ap.add_argument("--masks", type=int, default=None)
     ap.add_argument("--strip", type=int, default=None)
     ap.add_argument("--threads", type=int, default=None)
+    ap.add_argument("--layout", choices=("auto", "strips", "rows"), default=None,
+                    help="sieve: lane layout of the kernel (speed only)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,6 +121,8 @@ def main() -> int:
         tune["mi_strip_target"] = a.strip
     if a.threads is not None:
         tune["threads"] = a.threads
+    if a.layout is not None:
+        tune["mi_layout"] = a.layout
     plan = es_plan.Plan(K, tune, WHEEL)
     eng = G.GpuSieve(plan)
     N = 1 << a.window_log2

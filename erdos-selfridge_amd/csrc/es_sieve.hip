@@ -72,6 +72,113 @@ KernelFn pick_kernel(int F, uint32_t W, uint32_t G, uint32_t C) {
   if (F < kPartFirstF[3]) return pick_kernel_p2(F, W, G, C);
   return pick_kernel_p3(F, W, G, C);
 }
+RowsFn pick_rows(int F, uint32_t C) {
+  if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  if (F < kPartFirstF[1]) return pick_rows_p0(F, C);
+  if (F < kPartFirstF[2]) return pick_rows_p1(F, C);
+  if (F < kPartFirstF[3]) return pick_rows_p2(F, C);
+  return pick_rows_p3(F, C);
+}
+RowsBatchFn pick_rows_batch(int F, uint32_t C) {
+  if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  if (F < kPartFirstF[1]) return pick_rows_batch_p0(F, C);
+  if (F < kPartFirstF[2]) return pick_rows_batch_p1(F, C);
+  if (F < kPartFirstF[3]) return pick_rows_batch_p2(F, C);
+  return pick_rows_batch_p3(F, C);
+}
+
+// Lane slots of the strip rows for the row-lane layout: G groups of 64 slots, i.e. 2G
+// half-waves of 32.  bank[sg * F + f] is the bank pair (row mod 32) of strip row sg in mask
+// table f; a half-wave costs, per table, its most crowded bank pair (the cycles one
+// ds_read_b64 of those 32 lanes takes).  Rows are placed greedily (each to the open half-wave
+// it crowds least, in a seeded shuffled order), then pairs of slots in different half-waves
+// are swapped while the swap does not raise the two half-waves' cost.  Empty slots repeat a
+// row of their half-wave (the same address: no extra bank load) and carry bit 31 (no output).
+std::vector<uint32_t> group_strip_rows(const std::vector<uint32_t>& bank, uint32_t S, uint32_t F,
+                                       uint32_t G) {
+  const uint32_t H = 2 * G;                      // half-waves
+  const uint32_t FF = std::max<uint32_t>(F, 1);
+  std::vector<uint8_t> cnt((size_t)H * FF * 32, 0);
+  std::vector<uint32_t> slot((size_t)H * 32, UINT32_MAX), fill(H, 0);
+  auto bin = [&](uint32_t h, uint32_t f, uint32_t b) -> uint8_t& {
+    return cnt[((size_t)h * FF + f) * 32 + b];
+  };
+  auto cost = [&](uint32_t h) {
+    uint32_t c = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+      uint8_t mx = 0;
+      for (uint32_t b = 0; b < 32; ++b) mx = std::max(mx, bin(h, f, b));
+      c += mx;
+    }
+    return c;
+  };
+  uint64_t rng = 0x9e3779b97f4a7c15ull;
+  auto next = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  std::vector<uint32_t> order(S);
+  for (uint32_t i = 0; i < S; ++i) order[i] = i;
+  for (uint32_t i = S; i > 1; --i) std::swap(order[i - 1], order[next() % i]);
+  for (uint32_t r : order) {
+    uint32_t best_h = UINT32_MAX, best_add = UINT32_MAX;
+    for (uint32_t h = 0; h < H && best_add; ++h) {
+      if (fill[h] >= 32) continue;
+      uint32_t add = 0;
+      for (uint32_t f = 0; f < F; ++f) {
+        const uint8_t c = bin(h, f, bank[(size_t)r * F + f]);
+        uint8_t mx = 0;
+        for (uint32_t b = 0; b < 32; ++b) mx = std::max(mx, bin(h, f, b));
+        add += c + 1u > mx ? 1u : 0u;
+      }
+      if (add < best_add) {
+        best_add = add;
+        best_h = h;
+      }
+    }
+    slot[(size_t)best_h * 32 + fill[best_h]++] = r;
+    for (uint32_t f = 0; f < F; ++f) ++bin(best_h, f, bank[(size_t)r * F + f]);
+  }
+  const uint64_t iters = std::min<uint64_t>(60000, (uint64_t)S * 60);
+  const uint32_t n_slots = H * 32;
+  for (uint64_t it = 0; it < iters && H > 1; ++it) {
+    const uint32_t a = (uint32_t)(next() % n_slots), b = (uint32_t)(next() % n_slots);
+    const uint32_t ha = a / 32, hb = b / 32;
+    const uint32_t ra = slot[a], rb = slot[b];
+    if (ha == hb || (ra == UINT32_MAX && rb == UINT32_MAX)) continue;
+    const uint32_t before = cost(ha) + cost(hb);
+    auto move = [&](uint32_t r, uint32_t from, uint32_t to) {
+      if (r == UINT32_MAX) return;
+      for (uint32_t f = 0; f < F; ++f) {
+        --bin(from, f, bank[(size_t)r * F + f]);
+        ++bin(to, f, bank[(size_t)r * F + f]);
+      }
+    };
+    move(ra, ha, hb);
+    move(rb, hb, ha);
+    if (cost(ha) + cost(hb) <= before) {
+      slot[a] = rb;
+      slot[b] = ra;
+    } else {
+      move(ra, hb, ha);
+      move(rb, ha, hb);
+    }
+  }
+  std::vector<uint32_t> perm((size_t)G * 64, 0);
+  for (uint32_t h = 0; h < H; ++h) {
+    uint32_t any = UINT32_MAX;
+    for (uint32_t i = 0; i < 32 && any == UINT32_MAX; ++i) any = slot[(size_t)h * 32 + i];
+    if (any == UINT32_MAX) any = 0;                // an empty half-wave (S small): row 0
+    for (uint32_t i = 0; i < 32; ++i) {
+      const uint32_t r = slot[(size_t)h * 32 + i];
+      perm[(size_t)h * 32 + i] = r == UINT32_MAX ? (any | 0x80000000u) : r;
+    }
+  }
+  return perm;
+}
+
 GridBatchFn pick_grid_batch(int F, uint32_t W, uint32_t G, uint32_t C) {
   if (F < 0 || F > kMaxMasks) return nullptr;
   if (F < kPartFirstF[1]) return pick_grid_batch_p0(F, W, G, C);
@@ -118,6 +225,11 @@ struct es_engine {
   HotPlan hp{};
   ColdPlan cold{};
   KernelFn kern = nullptr;
+  bool rows = false;         // row-lane layout (lanes over strip rows, waves over steps)
+  uint32_t V = 1, Gl = 0;    // row-lane: steps per unit, lane groups of 64 strip rows
+  RowPlan rp{};
+  RowPlan* d_rp = nullptr;   // device copy of rp (the row-lane kernel takes a pointer)
+  RowsFn rows_kern = nullptr;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   uint64_t launches = 0, positions = 0;
@@ -152,13 +264,40 @@ void outer_range(const es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t*
   *o_hi = idx_hi > idx_lo ? ((idx_hi - 1) / per + 1) * e->lead : *o_lo;
 }
 
+// One kernel launch over outer indices [o_lo, o_hi) on stream st; a non-null `cold` replaces
+// the strip-lane plan's exact-test record, and a non-null `rows_plan` is the device copy of
+// the row-lane plan to use (the static-scratch scan).  Strip-lane: one lane per strip;
+// row-lane: one wave per unit (lane group, superstrip).
+int launch_kernel(es_engine* e, hipStream_t st, const ColdPlan* cold, const RowPlan* rows_plan,
+                  uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
+  if (e->rows) {
+    const uint64_t span = (uint64_t)e->S * e->V;
+    uint64_t w_lo = o_lo / span, w_hi = (o_hi + span - 1) / span;
+    const uint64_t units = (uint64_t)e->Gl * (w_hi - w_lo);
+    const uint32_t wpg = e->threads / 64;
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((units + wpg - 1) / wpg,
+                                                                   e->max_grid));
+    const RowPlan* rp = rows_plan ? rows_plan : e->d_rp;
+    void* args[] = {&rp, &w_lo, &w_hi, &o_lo, &o_hi, &idx_lo, &idx_hi};
+    ES_HIP(hipLaunchKernel((const void*)e->rows_kern, dim3((uint32_t)grid), dim3(e->threads),
+                           args, e->lds_bytes, st));
+  } else {
+    uint64_t u_lo = o_lo / e->S, u_hi = (o_hi + e->S - 1) / e->S;
+    const uint64_t grid = std::max<uint64_t>(
+        1, std::min<uint64_t>((u_hi - u_lo + e->threads - 1) / e->threads, e->max_grid));
+    HotPlan hp = e->hp;
+    if (cold) hp.cold = cold;
+    void* args[] = {&hp, &u_lo, &u_hi, &o_lo, &o_hi, &idx_lo, &idx_hi};
+    ES_HIP(hipLaunchKernel((const void*)e->kern, dim3((uint32_t)grid), dim3(e->threads), args,
+                           e->lds_bytes, st));
+  }
+  ES_HIP(hipGetLastError());
+  return ES_OK;
+}
+
 int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
   e->count_valid = false;
   if (o_hi <= o_lo) return ES_OK;
-  uint64_t u_lo = o_lo / e->S, u_hi = (o_hi + e->S - 1) / e->S;
-  uint64_t grid = (u_hi - u_lo + e->threads - 1) / e->threads;
-  if (grid > e->max_grid) grid = e->max_grid;
-  if (grid == 0) grid = 1;
   hipEvent_t a = nullptr, b = nullptr;
   if (e->ev_used + 2 <= e->ev_pool.size()) {
     a = e->ev_pool[e->ev_used];
@@ -166,10 +305,8 @@ int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, ui
     e->ev_used += 2;
     ES_HIP(hipEventRecord(a, e->stream));
   }
-  void* args[] = {&e->hp, &u_lo, &u_hi, &o_lo, &o_hi, &idx_lo, &idx_hi};
-  ES_HIP(hipLaunchKernel((const void*)e->kern, dim3((uint32_t)grid), dim3(e->threads), args,
-                         e->lds_bytes, e->stream));
-  ES_HIP(hipGetLastError());
+  const int rc = launch_kernel(e, e->stream, nullptr, nullptr, o_lo, o_hi, idx_lo, idx_hi);
+  if (rc) return rc;
   if (b) {
     ES_HIP(hipEventRecord(b, e->stream));
     // idx covered by outer indices [o_lo, o_hi) (multiples of lead when rings are swapped)
@@ -326,7 +463,22 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   e->S = (uint32_t)S;
   e->R_outer = (uint64_t)(R / e->unit);
   e->strips = (uint64_t)(R / r0) / e->S;
-  const uint32_t nb = w - 1 - m;
+  // Row-lane layout: the rings after the strip rings, while their radix product stays at
+  // most 1024, are the sub rings a wave steps through (V steps per unit: short enough that a
+  // window holds many units, so the last round of waves is short); the rest are the
+  // superstrip rings.  It applies to one-word rows with every mask in LDS, a strip of at
+  // least 32 rows and at least 16 steps.
+  if (d->layout > 2) return fail(ES_ERR_INVALID, "layout must be 0, 1 or 2");
+  uint32_t n_sub = 0;
+  uint64_t V = 1;
+  while (1 + m + n_sub < w && V * d->ring_radix[1 + m + n_sub] <= 1024) V *= d->ring_radix[1 + m + n_sub++];
+  bool rows = d->layout != 1 && W == 1 && d->global_masks == 0 && S >= 32 && V >= 16;
+  if (d->layout == 2 && !rows)
+    return fail(ES_ERR_LIMIT, "row-lane layout needs one-word rows, every mask in LDS, a strip "
+                              "of at least 32 rows and at least 16 steps");
+  e->rows = rows;
+  e->V = rows ? (uint32_t)V : 1;
+  e->Gl = rows ? (uint32_t)((S + 63) / 64) : 0;
 
   // ---- mask tables: rows [0, 2m) of 2W words (per ring-0 word: slots without / with an inner
   // wrap past M), one bit per sorted ring-0 slot.  Staged masks go to LDS, except that for
@@ -354,8 +506,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   std::vector<uint32_t> staged;  // indices into the descriptor's mask list
   {
     size_t used = carry_words + 8;
-    for (uint32_t f = 0; f < d->n_masks && staged.size() < (size_t)kMaxMasks; ++f) {
-      const size_t rows_words = (size_t)4 * d->mask_mod[f] * W + 4;
+    const size_t cap_masks = rows ? (size_t)kRowsMaxMasks : (size_t)kMaxMasks;
+    for (uint32_t f = 0; f < d->n_masks && staged.size() < cap_masks; ++f) {
+      const size_t rows_words = rows ? (size_t)2 * (3 * (size_t)d->mask_mod[f] + 64) + 4
+                                     : (size_t)4 * d->mask_mod[f] * W + 4;
       if (used + rows_words > budget_words) continue;  // stays exact-only
       if (inv_mod((uint32_t)((d->mask_mod[f] - mod_u128(M, d->mask_mod[f])) % d->mask_mod[f]),
                   d->mask_mod[f]) == 0)
@@ -364,6 +518,15 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
       staged.push_back(f);
     }
   }
+  if (rows && staged.empty()) {  // no mask tables: the strip-lane kernel
+    if (d->layout == 2) return fail(ES_ERR_LIMIT, "row-lane layout needs at least one mask");
+    rows = false;
+    e->rows = false;
+    e->V = 1;
+    e->Gl = 0;
+  }
+  const uint32_t nb = rows ? w - 1 - m - n_sub : w - 1 - m;
+  const uint32_t first_base = rows ? 1 + m + n_sub : 1 + m;
   uint32_t G = d->global_masks;
   if (W != 1 || (G != 2 && G != 4) || staged.size() < G) G = 0;
   // the G smallest staged tables go global, kept in their relative order at the end
@@ -379,27 +542,47 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     if (std::find(glob.begin(), glob.end(), f) == glob.end()) order_masks.push_back(f);
   for (uint32_t f : staged)
     if (std::find(glob.begin(), glob.end(), f) != glob.end()) order_masks.push_back(f);
+  // row-lane: an even modulus (a power of 2: at most one) goes last, with its own row map
+  uint32_t last_even = 0;
+  if (rows) {
+    auto it = std::find_if(order_masks.begin(), order_masks.end(),
+                           [&](uint32_t f) { return d->mask_mod[f] % 2 == 0; });
+    if (it != order_masks.end()) {
+      const uint32_t f = *it;
+      order_masks.erase(it);
+      order_masks.push_back(f);
+      last_even = 1;
+    }
+  }
   std::vector<uint32_t> lds, gimg, mbase, mmod, msrc, minv;
+  if (rows) lds.assign(carry_words, 0);  // row-lane: the carry table opens the image
   for (size_t pos = 0; pos < order_masks.size(); ++pos) {
     const uint32_t f = order_masks[pos];
     const bool in_global = pos + G >= order_masks.size();
     std::vector<uint32_t>& img = in_global ? gimg : lds;
     const uint32_t mq = d->mask_mod[f], src = d->mask_src[f];
-    const size_t rows_words = (size_t)2 * mq * 2 * W;
+    const uint32_t corr = (mq - mod_u128(M, mq)) % mq;
+    // row map: strip-lane keeps phase x at row x * (-M)^-1 (one row on per M subtracted), over
+    // rows [0, 2m); row-lane at row x * lambda, lambda = 32 (-M)^-1 (32 rows on per M
+    // subtracted: the same bank), over rows [0, 3m + 64)
+    const uint32_t cinv = inv_mod(corr, mq);
+    const uint32_t lam = rows && mq % 2 ? (uint32_t)((uint64_t)32 * cinv % mq) : cinv;
+    const uint32_t xstep = inv_mod(lam, mq);            // phase of row 1
+    const size_t n_rows = rows ? (size_t)3 * mq + 64 : (size_t)2 * mq;
+    const size_t rows_words = n_rows * 2 * W;
     const size_t off = (img.size() + 3) & ~size_t(3);  // 16-byte aligned rows
     img.resize(off + rows_words, 0);
     const uint32_t* bm = d->mask_bits + d->mask_off[f];
-    const uint32_t corr = (mq - mod_u128(M, mq)) % mq;
     std::vector<uint32_t> t0m(r0);
     for (uint32_t i = 0; i < r0; ++i) t0m[i] = mod_u128(e->terms[order[i]], mq);
     for (uint32_t P = 0; P < mq; ++P) {
-      const uint32_t x = (uint32_t)((uint64_t)P * corr % mq);  // phase held at row P
+      const uint32_t x = (uint32_t)((uint64_t)P * xstep % mq);  // phase held at row P
       for (uint32_t i = 0; i < r0; ++i) {
         const uint32_t a = (uint32_t)(((uint64_t)x + t0m[i]) % mq);
         const uint32_t b = (uint32_t)(((uint64_t)a + corr) % mq);
         const uint32_t wi = i >> 5, bit = 1u << (i & 31);
-        for (uint32_t copy = 0; copy < 2; ++copy) {
-          uint32_t* row = &img[off + (size_t)2 * W * (P + copy * mq) + 2 * wi];
+        for (size_t r = P; r < n_rows; r += mq) {
+          uint32_t* row = &img[off + (size_t)2 * W * r + 2 * wi];
           if ((bm[a >> 5] >> (a & 31)) & 1u) row[0] |= bit;
           if ((bm[b >> 5] >> (b & 31)) & 1u) row[1] |= bit;
         }
@@ -408,15 +591,15 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     mbase.push_back((uint32_t)(off * 4));
     mmod.push_back(mq);
     msrc.push_back(src);
-    minv.push_back(inv_mod(corr, mq));
+    minv.push_back(lam);
   }
   if (gimg.empty()) gimg.push_back(0);
   e->F = (uint32_t)mmod.size();
   e->G = G;
   e->mask_mod = mmod;
   e->mask_src = msrc;
-  const uint32_t carry_off = (uint32_t)((lds.size() + 3) & ~size_t(3));
-  lds.resize(carry_off + carry_words, 0);
+  const uint32_t carry_off = rows ? 0u : (uint32_t)((lds.size() + 3) & ~size_t(3));
+  if (!rows) lds.resize(carry_off + carry_words, 0);
   for (uint32_t h = 0; h < q0; ++h) {
     const uint32_t j = (uint32_t)(std::lower_bound(e_sorted.begin(), e_sorted.begin() + r0,
                                                    q0 - h) - e_sorted.begin());
@@ -455,12 +638,15 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   }
 
   // ---- base-ring terms in (h, l) form
+  if (rows && nb > 64) return fail(ES_ERR_LIMIT, "more than 64 superstrip rings");
   std::vector<uint32_t> b_radix(std::max<uint32_t>(nb, 1), 0), b_off(std::max<uint32_t>(nb, 1), 0), t_h;
+  std::vector<uint64_t> b_prefix(std::max<uint32_t>(nb, 1), 1);
   std::vector<uint64_t> t_l;
   for (uint32_t j = 0; j < nb; ++j) {
-    const uint32_t ring = 1 + m + j;
+    const uint32_t ring = first_base + j;
     b_radix[j] = d->ring_radix[ring];
     b_off[j] = (uint32_t)t_h.size();
+    if (j > 0) b_prefix[j] = b_prefix[j - 1] * b_radix[j - 1];
     for (uint32_t i = term_off[ring]; i < term_off[ring + 1]; ++i) {
       const u128 tv = e->terms[i];
       t_h.push_back((uint32_t)(tv / e->M0));
@@ -505,6 +691,37 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
       if (++dig[j] < d->ring_radix[1 + j]) break;
       dig[j] = 0;
     }
+  }
+
+  // ---- row-lane: step table over the sub rings (ring 1 + m fastest) and the lane groups
+  std::vector<uint32_t> steps, perm;
+  if (rows) {
+    steps.assign((size_t)e->V * kStepDwords, 0);
+    std::vector<uint32_t> sd(n_sub, 0);
+    for (uint32_t v = 0; v < e->V; ++v) {
+      u128 B = 0;
+      for (uint32_t j = 0; j < n_sub; ++j) {
+        B += e->terms[term_off[1 + m + j] + sd[j]];
+        if (B >= M) B -= M;
+      }
+      uint32_t* st = &steps[(size_t)v * kStepDwords];
+      st[0] = (uint32_t)(B / e->M0);
+      const u128 z = e->M0 - B % e->M0;              // in (0, M0]
+      st[1] = (uint32_t)z;
+      st[2] = (uint32_t)(z >> 32);
+      st[3] = (uint32_t)(z >> 64);
+      st[4] = (uint32_t)(z >> 96);
+      for (uint32_t f = 0; f < F; ++f)
+        st[5 + f] = 8u * (uint32_t)((uint64_t)mod_u128(B, mmod[f]) * minv[f] % mmod[f]);
+      for (uint32_t j = 0; j < n_sub; ++j) {  // odometer
+        if (++sd[j] < d->ring_radix[1 + m + j]) break;
+        sd[j] = 0;
+      }
+    }
+    std::vector<uint32_t> bank((size_t)S * std::max<uint32_t>(F, 1), 0);
+    for (uint64_t sg = 0; sg < S; ++sg)
+      for (uint32_t f = 0; f < F; ++f) bank[sg * F + f] = (strip[sg * stride + 8 + f] / 8u) & 31u;
+    perm = group_strip_rows(bank, (uint32_t)S, F, e->Gl);
   }
 
   // ---- mask records
@@ -554,14 +771,25 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   const size_t o_ids = bl.put(inner_ids.data(), inner_ids.size() * 4);
   const size_t o_slow = bl.put(e->slow_info.data(), e->slow_info.size() * 4);
   const size_t o_sbits = bl.put(fbits.data(), fbits.size() * 4);
+  const size_t o_bpre = bl.put(b_prefix.data(), b_prefix.size() * 8);
+  const size_t o_rp = bl.put(nullptr, sizeof(RowPlan));
+  const size_t o_steps = bl.put(steps.data(), steps.size() * 4);
+  const size_t o_perm = bl.put(perm.data(), perm.size() * 4);
   const size_t o_cold = bl.put(nullptr, sizeof(ColdPlan));
   const size_t o_cnt = bl.put(nullptr, 24);
 
   ES_HIP(hipGetDevice(&e->device));
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
-  e->kern = pick_kernel((int)e->F, W, G, e->kind);
-  if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
+  if (rows) {
+    e->rows_kern = pick_rows((int)e->F, e->kind);
+    if (!e->rows_kern) return fail(ES_ERR_LIMIT, "no row-lane kernel for this mask count");
+    e->kern = nullptr;
+  } else {
+    e->kern = pick_kernel((int)e->F, W, G, e->kind);
+    if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
+  }
+  const void* kfn = rows ? (const void*)e->rows_kern : (const void*)e->kern;
   if (!e->threads) {
     // The workgroup size that keeps the most waves resident per CU (ties: the smaller one).
     // Every workgroup holds its own copy of the tables, so large tables favour larger
@@ -571,12 +799,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
       const uint32_t lds_t = table_bytes + (t / 64) * kWaveQueueBytes;
       if (lds_t > prop.sharedMemPerBlock) continue;
       if (lds_t > 65536)
-        ES_HIP(hipFuncSetAttribute((const void*)e->kern,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+        ES_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)prop.sharedMemPerBlock));
       int nb = 0;
-      ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)e->kern, (int)t,
-                                                          lds_t));
+      ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, (int)t, lds_t));
       if ((uint32_t)nb * (t / 64) > best_waves) {
         best_waves = (uint32_t)nb * (t / 64);
         e->threads = t;
@@ -639,8 +865,36 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   P.gmask = (const uint32_t*)(base + o_gm);
   P.cold = (const ColdPlan*)(base + o_cold);
   if (e->lds_bytes > 65536)  // the device's limit, so engines sharing a kernel never lower it
-    ES_HIP(hipFuncSetAttribute((const void*)e->kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+    ES_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)prop.sharedMemPerBlock));
+  if (rows) {
+    RowPlan& Q = e->rp;
+    Q.M0_lo = P.M0_lo;
+    Q.M0_hi = P.M0_hi;
+    Q.q0 = q0;
+    Q.r0 = r0;
+    Q.S = e->S;
+    Q.G = e->Gl;
+    Q.V = e->V;
+    Q.nb = nb;
+    Q.lds_words = P.lds_words;
+    Q.carry_base = P.carry_base;
+    Q.stride = stride;
+    Q.last_even = last_even;
+    Q.lds_src = P.lds_src;
+    Q.perm = (const uint32_t*)(base + o_perm);
+    Q.strip = P.strip;
+    Q.steps = (const uint32_t*)(base + o_steps);
+    Q.base_radix = P.base_radix;
+    Q.base_off = P.base_off;
+    Q.base_prefix = (const uint64_t*)(base + o_bpre);
+    Q.term_h = P.term_h;
+    Q.term_l = P.term_l;
+    Q.masks = P.masks;
+    Q.cold = P.cold;
+    e->d_rp = (RowPlan*)(base + o_rp);
+    ES_HIP(hipMemcpy(e->d_rp, &Q, sizeof(RowPlan), hipMemcpyHostToDevice));
+  }
   e->launch_outer = std::max<uint64_t>(1, ((uint64_t)1 << (d->launch_log2 ? d->launch_log2 : 37)) / r0);
   e->ev_pool.resize(512);
   for (auto& ev : e->ev_pool) ES_HIP(hipEventCreate(&ev));
@@ -674,6 +928,9 @@ int es_engine_info_get(const es_engine* e, es_engine_info* info) {
   info->strip_len = e->S;
   info->n_masks_used = e->F;
   info->global_masks = e->G;
+  info->layout = e->rows ? 2 : 1;
+  info->steps = e->rows ? e->V : 0;
+  info->groups = e->rows ? e->Gl : 0;
   info->lds_bytes = e->lds_bytes;
   info->words = e->W;
   info->threads = e->threads;
@@ -834,6 +1091,7 @@ struct es_batch {
     unsigned long long* d_out = nullptr;
     unsigned long long* d_count = nullptr;
     HotPlan hp{};
+    RowPlan rp{};                           // row-lane engines
   };
   hipStream_t stream = nullptr;
   // groups of different shapes are independent: they run concurrently on side streams
@@ -846,7 +1104,7 @@ struct es_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* d_tables = nullptr;  // plan views and job table of the last run (grown as needed)
   size_t tables_bytes = 0;
-  size_t tables_plan_bytes = 0, tables_job_bytes = 0;
+  size_t tables_hot_bytes = 0, tables_plan_bytes = 0, tables_job_bytes = 0;
   hipGraph_t graph = nullptr;          // captured launches (es_batch_capture)
   hipGraphExec_t graph_exec = nullptr;
   void drop_graph() {
@@ -962,6 +1220,9 @@ int es_batch_add(es_batch* b, es_engine* e, uint64_t t, uint64_t idx_lo, uint64_
   j.hp = e->hp;
   j.hp.masks = (const uint32_t*)(base + o_mi);
   j.hp.cold = (const ColdPlan*)(base + o_cold);
+  j.rp = e->rp;
+  j.rp.masks = j.hp.masks;
+  j.rp.cold = j.hp.cold;
   if (job_id) *job_id = (uint32_t)b->jobs.size();
   b->jobs.push_back(j);
   b->drop_graph();  // a captured graph no longer covers every job
@@ -1011,7 +1272,12 @@ namespace {
 // Launch groups of a batch with their device tables uploaded (prepare_batch), and the launch
 // of every group on the batch's stream (launch_batch: no allocation or host synchronisation,
 // so it can be captured into a graph).
-struct BatchLaunch { GridBatchFn fn; uint32_t threads, plan_off, job_off, n, lds_max; uint64_t wgs; };
+struct BatchLaunch {
+  const void* fn;
+  bool rows;
+  uint32_t threads, plan_off, job_off, n, lds_max;
+  uint64_t wgs;
+};
 
 int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   out.clear();
@@ -1020,11 +1286,12 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   // (2048 lanes and 160 KiB of LDS per CU) share one; a small plan is never held to a large
   // plan's occupancy.
   std::vector<std::vector<uint32_t>> groups;
-  struct Key { GridBatchFn fn; uint32_t threads, occ; };
+  struct Key { const void* fn; uint32_t threads, occ; };
   std::vector<Key> keys;
   for (uint32_t i = 0; i < b->jobs.size(); ++i) {
     const es_engine* e = b->jobs[i].e;
-    const GridBatchFn fn = pick_grid_batch((int)e->F, e->W, 0, e->kind);
+    const void* fn = e->rows ? (const void*)pick_rows_batch((int)e->F, e->kind)
+                             : (const void*)pick_grid_batch((int)e->F, e->W, 0, e->kind);
     if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
     const uint32_t occ = std::min<uint32_t>(2048u / e->threads,
                                             163840u / std::max<uint32_t>(e->lds_bytes, 1));
@@ -1038,38 +1305,57 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
     }
     groups[g].push_back(i);
   }
-  // per group: plan views [plan_off, +n) and job entries [job_off, +n) in one device table,
-  // then the jobs' counter addresses
-  struct Launch { uint32_t plan_off, job_off, n, lds_max; uint64_t wgs; };
+  // per group: plan views [plan_off, +n) (strip-lane groups in the HotPlan array, row-lane
+  // groups in the RowPlan array) and job entries [job_off, +n) in one device table, then the
+  // jobs' counter addresses
+  struct Launch { bool rows; uint32_t plan_off, job_off, n, lds_max; uint64_t wgs; };
   std::vector<Launch> runs;
   std::vector<HotPlan> plans;
+  std::vector<RowPlan> rplans;
   std::vector<GridJob> gj;
   for (size_t g = 0; g < groups.size(); ++g) {
-    // every job's strips as consecutive workgroups of one launch
+    // every job's strips (strip-lane) or units (row-lane: one per wave) as consecutive
+    // workgroups of one launch
     const uint32_t threads = keys[g].threads;
-    Launch L{(uint32_t)plans.size(), (uint32_t)gj.size(), (uint32_t)groups[g].size(), 4, 0};
+    const bool rows = b->jobs[groups[g][0]].e->rows;
+    Launch L{rows, (uint32_t)(rows ? rplans.size() : plans.size()), (uint32_t)gj.size(),
+             (uint32_t)groups[g].size(), 4, 0};
     uint64_t wgs = 0;
     for (uint32_t ji : groups[g]) {
       const es_batch::Job& j = b->jobs[ji];
       const es_engine* e = j.e;
       GridJob J{};
-      J.plan = (uint32_t)(plans.size() - L.plan_off);
-      plans.push_back(j.hp);
+      if (rows) {
+        J.plan = (uint32_t)(rplans.size() - L.plan_off);
+        rplans.push_back(j.rp);
+      } else {
+        J.plan = (uint32_t)(plans.size() - L.plan_off);
+        plans.push_back(j.hp);
+      }
       L.lds_max = std::max(L.lds_max, e->lds_bytes);
       outer_range(e, j.idx_lo, j.idx_hi, &J.o_lo, &J.o_hi);
-      J.u_lo = J.o_lo / e->S;
-      J.u_hi = (J.o_hi + e->S - 1) / e->S;
       J.idx_lo = j.idx_lo;
       J.idx_hi = j.idx_hi;
       J.wg_start = wgs;
-      wgs += (J.u_hi - J.u_lo + threads - 1) / threads;
+      if (rows) {
+        const uint64_t span = (uint64_t)e->S * e->V;
+        J.u_lo = J.o_lo / span;
+        J.u_hi = (J.o_hi + span - 1) / span;
+        wgs += ((uint64_t)e->Gl * (J.u_hi - J.u_lo) + threads / 64 - 1) / (threads / 64);
+      } else {
+        J.u_lo = J.o_lo / e->S;
+        J.u_hi = (J.o_hi + e->S - 1) / e->S;
+        wgs += (J.u_hi - J.u_lo + threads - 1) / threads;
+      }
       gj.push_back(J);
     }
     if (wgs > 0x7fffffffull) return fail(ES_ERR_LIMIT, "batch too large for one grid");
     L.wgs = wgs;
     runs.push_back(L);
   }
-  const size_t plan_bytes = (plans.size() * sizeof(HotPlan) + 255) & ~size_t(255);
+  const size_t hot_bytes = (plans.size() * sizeof(HotPlan) + 255) & ~size_t(255);
+  const size_t row_bytes = (rplans.size() * sizeof(RowPlan) + 255) & ~size_t(255);
+  const size_t plan_bytes = hot_bytes + row_bytes;
   const size_t job_bytes = (gj.size() * sizeof(GridJob) + 255) & ~size_t(255);
   std::vector<unsigned long long*> ctrs;
   for (auto& j : b->jobs) ctrs.push_back(j.d_count);
@@ -1082,9 +1368,13 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
     b->tables_bytes = need;
   }
   uint8_t* tb = (uint8_t*)b->d_tables;
-  if (!plans.empty()) {
-    ES_HIP(hipMemcpyAsync(tb, plans.data(), plans.size() * sizeof(HotPlan), hipMemcpyHostToDevice,
-                          b->stream));
+  if (!gj.empty()) {
+    if (!plans.empty())
+      ES_HIP(hipMemcpyAsync(tb, plans.data(), plans.size() * sizeof(HotPlan),
+                            hipMemcpyHostToDevice, b->stream));
+    if (!rplans.empty())
+      ES_HIP(hipMemcpyAsync(tb + hot_bytes, rplans.data(), rplans.size() * sizeof(RowPlan),
+                            hipMemcpyHostToDevice, b->stream));
     ES_HIP(hipMemcpyAsync(tb + plan_bytes, gj.data(), gj.size() * sizeof(GridJob),
                           hipMemcpyHostToDevice, b->stream));
     ES_HIP(hipMemcpyAsync(tb + plan_bytes + job_bytes, ctrs.data(), ctrs.size() * 8,
@@ -1097,11 +1387,12 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
       int dev = 0, lim = 0;
       ES_HIP(hipGetDevice(&dev));
       ES_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-      ES_HIP(hipFuncSetAttribute((const void*)keys[g].fn,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+      ES_HIP(hipFuncSetAttribute(keys[g].fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
     }
-    out.push_back({keys[g].fn, keys[g].threads, L.plan_off, L.job_off, L.n, L.lds_max, L.wgs});
+    out.push_back({keys[g].fn, L.rows, keys[g].threads, L.plan_off, L.job_off, L.n, L.lds_max,
+                   L.wgs});
   }
+  b->tables_hot_bytes = hot_bytes;
   b->tables_plan_bytes = plan_bytes;
   b->tables_job_bytes = job_bytes;
   return ES_OK;
@@ -1127,7 +1418,8 @@ int launch_batch(es_batch* b, const std::vector<BatchLaunch>& runs, bool zero_ke
     const BatchLaunch& L = runs[g];
     const int si = (int)(g % (size_t)ns);
     hipStream_t st = si == 0 ? b->stream : b->side[si - 1];
-    const HotPlan* pp = (const HotPlan*)tb + L.plan_off;
+    const void* pp = L.rows ? (const void*)((const RowPlan*)(tb + b->tables_hot_bytes) + L.plan_off)
+                            : (const void*)((const HotPlan*)tb + L.plan_off);
     const GridJob* jp = (const GridJob*)(tb + b->tables_plan_bytes) + L.job_off;
     uint32_t n = L.n;
     void* args[] = {&pp, &jp, &n};
@@ -1222,10 +1514,12 @@ namespace {
 
 constexpr uint64_t kStaticMaxIds = 65536;      // survivor slots the scratch records
 constexpr size_t kStaticColdOff = 0;           // ColdPlan of the call (the exact test's view)
-constexpr size_t kStaticCtrOff = 256;          // count, 1 + slot of the minimum, floor
-constexpr size_t kStaticRecOff = 512;          // [kStaticMaxIds][2] residues (lo, hi)
+constexpr size_t kStaticRowOff = 256;          // RowPlan of the call (row-lane engines)
+constexpr size_t kStaticCtrOff = 768;          // count, 1 + slot of the minimum, floor
+constexpr size_t kStaticRecOff = 1024;         // [kStaticMaxIds][2] residues (lo, hi)
 constexpr size_t kScratchBytes = kStaticRecOff + kStaticMaxIds * 16;
-static_assert(sizeof(ColdPlan) <= kStaticCtrOff - kStaticColdOff, "ColdPlan outgrew its slot");
+static_assert(sizeof(ColdPlan) <= kStaticRowOff - kStaticColdOff, "ColdPlan outgrew its slot");
+static_assert(sizeof(RowPlan) <= kStaticCtrOff - kStaticRowOff, "RowPlan outgrew its slot");
 
 }  // namespace
 
@@ -1251,12 +1545,14 @@ int scratch_address(int dev, unsigned char** out) {
   return ES_OK;
 }
 
-// The call's plan record, zeroed counters and the minimum's floor: written by a kernel (a
+// The call's plan records, zeroed counters and the minimum's floor: written by a kernel (a
 // copy or memset node would not be safe to capture).
-__global__ void es_static_setup_kernel(ColdPlan cold, unsigned char* __restrict__ base,
+__global__ void es_static_setup_kernel(ColdPlan cold, RowPlan rows, unsigned char* __restrict__ base,
                                        uint64_t floor) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *reinterpret_cast<ColdPlan*>(base + kStaticColdOff) = cold;
+    rows.cold = reinterpret_cast<const ColdPlan*>(base + kStaticColdOff);
+    *reinterpret_cast<RowPlan*>(base + kStaticRowOff) = rows;
     unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base + kStaticCtrOff);
     ctr[0] = 0;
     ctr[1] = 0;
@@ -1305,10 +1601,9 @@ extern "C" int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint6
   cold.out_s = reinterpret_cast<unsigned long long*>(base + kStaticRecOff);
   cold.out_cap = cap;
   const uint64_t floor = e->t == 0 ? (uint64_t)e->k + 2 : 0;
-  hipLaunchKernelGGL(es_static_setup_kernel, dim3(1), dim3(64), 0, st, cold, base, floor);
+  hipLaunchKernelGGL(es_static_setup_kernel, dim3(1), dim3(64), 0, st, cold, e->rp, base, floor);
   ES_HIP(hipGetLastError());
-  HotPlan hp = e->hp;
-  hp.cold = reinterpret_cast<const ColdPlan*>(base + kStaticColdOff);
+  const ColdPlan* cold_dev = reinterpret_cast<const ColdPlan*>(base + kStaticColdOff);
   if (idx_hi > idx_lo) {
     uint64_t o_lo, o_hi;
     outer_range(e, idx_lo, idx_hi, &o_lo, &o_hi);
@@ -1316,15 +1611,9 @@ extern "C" int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint6
     uint64_t per = (o_hi - o_lo + n - 1) / n;
     per = (per + e->lead - 1) / e->lead * e->lead;
     for (uint64_t a = o_lo; a < o_hi; a += per) {
-      uint64_t lo = a, hi = std::min(o_hi, a + per);
-      uint64_t u_lo = lo / e->S, u_hi = (hi + e->S - 1) / e->S;
-      uint64_t grid = (u_hi - u_lo + e->threads - 1) / e->threads;
-      grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, e->max_grid));
-      uint64_t wlo = idx_lo, whi = idx_hi;
-      void* args[] = {&hp, &u_lo, &u_hi, &lo, &hi, &wlo, &whi};
-      ES_HIP(hipLaunchKernel((const void*)e->kern, dim3((uint32_t)grid), dim3(e->threads), args,
-                             e->lds_bytes, st));
-      ES_HIP(hipGetLastError());
+      rc = launch_kernel(e, st, cold_dev, reinterpret_cast<const RowPlan*>(base + kStaticRowOff),
+                         a, std::min(o_hi, a + per), idx_lo, idx_hi);
+      if (rc) return rc;
     }
   }
   hipLaunchKernelGGL(es_static_finish_kernel, dim3(1), dim3(64), 0, st,

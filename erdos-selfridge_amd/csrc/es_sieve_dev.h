@@ -124,18 +124,12 @@ struct HotPlan {
 // makes the whole kernel allocate registers around the call: 96 VGPRs and spills against 79
 // and none inline for 6 masks, one word); wider rows, whose unrolled word loop has one drain
 // per word, call it out of line to keep the code small.
+// The core takes the outer index's residue b = h * M0 + l (h < q0, l < M0) directly.
 template <int CF>
-__device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
-                                           uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
-                                           uint64_t zh, uint32_t sg, uint64_t o,
-                                           uint64_t idx_lo, uint64_t idx_hi) {
-  const uint32_t* row = C->strip + (size_t)sg * C->stride;
-  const u128 L = es::mk128((uint64_t)row[2] | ((uint64_t)row[3] << 32),
-                       (uint64_t)row[4] | ((uint64_t)row[5] << 32));
+__device__ __forceinline__ void es_exact_test_hl(const ColdPlan* __restrict__ C, uint32_t bits,
+                                              uint32_t w, uint32_t h, u128 l, uint64_t o,
+                                              uint64_t idx_lo, uint64_t idx_hi) {
   const u128 M0 = es::mk128(C->M0_lo, C->M0_hi);
-  const u128 z = es::mk128(zl, zh);
-  u128 l = (M0 - z) + L;  // l_u + L_sigma
-  if (kap) l -= M0;
   const uint32_t q0 = C->q0, r0 = C->r0, ns = C->n_slow;
   const bool fp = C->slow_fp != 0;
   while (bits) {
@@ -177,6 +171,21 @@ __device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, ui
     es_emit(C, ok, idx, (uint64_t)(uint32_t)d0 | ((uint64_t)(uint32_t)d1 << 32),
             (uint64_t)(uint32_t)d2 | ((uint64_t)(uint32_t)d3 << 32));
   }
+}
+
+template <int CF>
+__device__ __forceinline__ void es_exact_test(const ColdPlan* __restrict__ C, uint32_t bits,
+                                           uint32_t w, uint32_t h, uint32_t kap, uint64_t zl,
+                                           uint64_t zh, uint32_t sg, uint64_t o,
+                                           uint64_t idx_lo, uint64_t idx_hi) {
+  const uint32_t* row = C->strip + (size_t)sg * C->stride;
+  const u128 L = es::mk128((uint64_t)row[2] | ((uint64_t)row[3] << 32),
+                       (uint64_t)row[4] | ((uint64_t)row[5] << 32));
+  const u128 M0 = es::mk128(C->M0_lo, C->M0_hi);
+  const u128 z = es::mk128(zl, zh);
+  u128 l = (M0 - z) + L;  // l_u + L_sigma
+  if (kap) l -= M0;
+  es_exact_test_hl<CF>(C, bits, w, h, l, o, idx_lo, idx_hi);
 }
 
 static __device__ __noinline__ void es_exact_test_call(const ColdPlan* __restrict__ C, uint32_t bits,
@@ -475,6 +484,335 @@ es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restric
   es_strip<F, G, W, C>(P, lds, in ? my : J.u_lo, in, J.o_lo, J.o_hi, J.idx_lo, J.idx_hi);
 }
 
+// ============================================================================ row-lane layout
+//
+// Lanes over strip rows.  The strip rows sg in [0, S) (rings 1..m) are grouped 64 to a wave,
+// chosen at engine creation so that within each half-wave of 32 lanes their rows of every mask
+// table fall in distinct bank pairs as far as possible (a ds_read_b64 serves 32 lanes a cycle
+// when their 8-byte rows sit in 32 distinct bank pairs).  A wave then walks V consecutive
+// steps v of the sub rings m+1..m+n under one superstrip base w of the remaining rings: the
+// step's values are one wave-uniform row of a step table (scalar loads), the superstrip's are
+// decoded once per unit of V steps.  Outer index o = sg + S * (v + V * w).
+//
+// Table layout.  Mask table f keeps phase x at row x * lambda mod m, lambda = 32 * (-M)^-1
+// mod m: subtracting M moves a phase 32 rows on, a whole round of the 64 banks (32 rows of 8
+// bytes), so the wraps past M never move a lane to another bank.  (A modulus 2^D has no such
+// lambda -- 32 is not invertible mod 2^D -- so a power-of-2 table, placed last, keeps
+// lambda = (-M)^-1: one row per M subtracted, and its lanes may change bank on a wrap.)  A lane's row is
+// Q_sg + W_w + 32 c_u + D_v + 32 c_v (strip row, superstrip base, the wrap of their sum at
+// unit set-up, step, the wrap at the step) without reduction, so rows [0, 3m + 64) are
+// stored; only Q_sg and c_u differ between the lanes of a wave, and c_u keeps the bank.
+constexpr int kRowsMaxMasks = 11;   // a step row: h, z (4 dwords), then one row offset per mask
+constexpr int kStepDwords = 16;
+
+struct RowPlan {
+  uint64_t M0_lo, M0_hi;
+  uint32_t q0, r0, S, G;           // ring-0 modulus and radix; strip rows; lane groups of 64
+  uint32_t V, nb;                  // steps per unit; superstrip rings
+  uint32_t lds_words, carry_base;  // LDS image size (words), carry table byte offset
+  uint32_t stride;                 // dwords per strip row
+  uint32_t last_even;              // 1: the last table's modulus is even (a power of 2): its
+                                   // rows map x -> x (-M)^-1 (one row per M subtracted)
+  const uint32_t* lds_src;         // device copy of the LDS image
+  const uint32_t* perm;            // [G * 64] strip row of each lane slot; bit 31: a padding
+                                   // lane (it repeats a row of its half-wave and emits nothing)
+  const uint32_t* strip;           // [S][stride] H, X, L (4 dwords), 0, 0, Q8_f (row bytes)
+  const uint32_t* steps;           // [V][16] h, z = M0 - l (4 dwords), D8_f (row bytes)
+  const uint32_t* base_radix;      // [nb]
+  const uint32_t* base_off;        // [nb] first term of each superstrip ring
+  const uint64_t* base_prefix;     // [nb] product of the radices of the rings before ring j
+  const uint32_t* term_h;          // superstrip-ring terms, h part
+  const uint64_t* term_l;          // superstrip-ring terms, l part (lo, hi pairs)
+  const uint32_t* masks;           // [F][8] m, table byte base, M0 mod m, 2^32, 2^64, 2^96 mod m,
+                                   //        lambda, t*M mod m
+  const ColdPlan* cold;
+};
+
+__device__ __forceinline__ uint2* es_rows_queue(const RowPlan& P, const uint32_t* lds) {
+  const uint32_t base = (P.lds_words * 4u + 15u) & ~15u;
+  return reinterpret_cast<uint2*>(reinterpret_cast<char*>(const_cast<uint32_t*>(lds)) + base +
+                                  (threadIdx.x >> 6) * kWaveQueueBytes);
+}
+
+// h and l of n's residue at step row `st` for a lane with superstrip-plus-strip-row sums
+// reduced at unit set-up (Hp < q0, Lp < M0): b = (Hp + h_D) M0 + Lp + l_D, reduced to h < q0;
+// c = 1 when M was subtracted.
+__device__ __forceinline__ void es_rows_split(uint32_t q0, uint32_t Hp, uint64_t Lpl, uint64_t Lph,
+                                              uint32_t hD, uint64_t zl, uint64_t zh,
+                                              uint32_t& h, uint32_t& c, bool& kap) {
+  kap = es_ge128(Lph, Lpl, zh, zl);          // Lp + l_D >= M0
+  h = Hp + hD + (kap ? 1u : 0u);             // < 2 q0
+  c = h >= q0 ? 1u : 0u;
+  h = c ? h - q0 : h;
+}
+
+// Exact re-test of queued entries [first, first + n) (one per lane), called by the whole wave;
+// an entry is {bits, v | lane << 26}: the entry's lane supplies its sums, strip row and
+// correction through cross-lane permutes, its step row is read by vector loads.
+template <int C>
+__device__ __forceinline__ void es_rows_drain(const RowPlan& P, const uint2* q, uint32_t first,
+                                              uint32_t n, uint32_t Hp, uint64_t Lpl, uint64_t Lph,
+                                              uint32_t sg, uint32_t X, uint64_t obase,
+                                              uint64_t idx_lo, uint64_t idx_hi) {
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint2 ent = lane < n ? q[first + lane] : make_uint2(0u, 0u);
+  const uint32_t src = ent.y >> kQueueIndexBits;
+  const uint64_t sLl = (uint64_t)es_from_lane(src, (uint32_t)Lpl) |
+                       ((uint64_t)es_from_lane(src, (uint32_t)(Lpl >> 32)) << 32);
+  const uint64_t sLh = (uint64_t)es_from_lane(src, (uint32_t)Lph) |
+                       ((uint64_t)es_from_lane(src, (uint32_t)(Lph >> 32)) << 32);
+  const uint32_t sH = es_from_lane(src, Hp), ssg = es_from_lane(src, sg), sX = es_from_lane(src, X);
+  if (lane < n) {
+    const uint32_t v = ent.y & ((1u << kQueueIndexBits) - 1u);
+    const uint32_t* st = P.steps + (size_t)v * kStepDwords;
+    const uint64_t zl = (uint64_t)st[1] | ((uint64_t)st[2] << 32);
+    const uint64_t zh = (uint64_t)st[3] | ((uint64_t)st[4] << 32);
+    uint32_t h, c;
+    bool kap;
+    es_rows_split(P.q0, sH, sLl, sLh, st[0], zl, zh, h, c, kap);
+    const u128 Lp = es::mk128(sLl, sLh), z = es::mk128(zl, zh);
+    const u128 M0 = es::mk128(P.M0_lo, P.M0_hi);
+    const u128 l = kap ? Lp - z : Lp + (M0 - z);   // Lp + l_D (mod M0)
+    const uint64_t o = obase + (uint64_t)v * P.S + ssg;
+    const uint64_t ob = (C & 2) ? o * P.r0 + (uint64_t)(int64_t)(int32_t)sX : o;
+    es_exact_test_hl<C>(P.cold, ent.x, 0, h, l, ob, idx_lo, idx_hi);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One unit: lane group g of the strip rows, superstrip w, all V steps.  Wave-uniform.
+template <int F, int C, bool CHECK>
+__device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* lds,
+                                              const uint32_t (&A)[F + 1], uint32_t Hp,
+                                              uint64_t Lpl, uint64_t Lph, uint32_t sg,
+                                              uint32_t X, bool valid, uint64_t obase,
+                                              uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo,
+                                              uint64_t idx_hi) {
+  const uint32_t q0 = P.q0, V = P.V, S = P.S;
+  const char* lb = reinterpret_cast<const char*>(lds);
+  // the carry table opens the LDS image (its address is h itself)
+  const uint8_t* carry_tab = reinterpret_cast<const uint8_t*>(lds);
+  const uint32_t* carry_w = lds;
+  uint2* wq = es_rows_queue(P, lds);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t vmask = __ballot(valid);
+  uint32_t cnt = 0;
+  for (uint32_t v = 0; v < V; ++v) {
+    cu32_t* st = (cu32_t*)(P.steps) + (size_t)v * kStepDwords;
+    const uint64_t zl = (uint64_t)st[1] | ((uint64_t)st[2] << 32);
+    const uint64_t zh = (uint64_t)st[3] | ((uint64_t)st[4] << 32);
+    uint32_t h, c;
+    bool kap;
+    es_rows_split(q0, Hp, Lpl, Lph, st[0], zl, zh, h, c, kap);
+    const uint32_t wrap = c ? 256u : 0u;      // 32 rows of 8 bytes when M was subtracted
+    const uint32_t wrap_last = c ? (P.last_even ? 8u : 256u) : 0u;
+    uint32_t bx = 0xffffffffu, by = 0xffffffffu;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const uint2 x = *reinterpret_cast<const uint2*>(lb + A[f] + st[5 + f] +
+                                                      (f == F - 1 ? wrap_last : wrap));
+      bx &= x.x;
+      by &= x.y;
+    }
+    uint32_t carry;
+    if (C & 1) {  // byte table of j(h) <= 32: bits i >= j carry (one 64-bit shift)
+      carry = (uint32_t)(0xffffffffull << carry_tab[h]);
+    } else {
+      carry = carry_w[h];
+    }
+    uint32_t bits = (carry & by) | (~carry & bx);
+    if (CHECK) {
+      const uint64_t o = obase + (uint64_t)v * S + sg;
+      if (o < o_lo || o >= o_hi) bits = 0;
+    }
+    // padding lanes (a repeated row) are left out through the wave's mask of valid lanes
+    const uint64_t any = __ballot(bits != 0u) & vmask;
+    if (any) {
+      if (bits && valid) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
+        wq[cnt + below] = make_uint2(bits, v | (lane << kQueueIndexBits));
+      }
+      cnt += (uint32_t)__popcll(any);
+      if (cnt >= 64u) {
+        cnt -= 64u;
+        es_rows_drain<C>(P, wq, cnt, 64u, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+      }
+    }
+  }
+  if (cnt) es_rows_drain<C>(P, wq, 0, cnt, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+}
+
+// Unit set-up: decode superstrip w, its table rows (block constant included), the lane's
+// strip row and sums; then sweep the V steps.
+template <int F, int C>
+__device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* lds, uint32_t g,
+                                             uint64_t w, uint64_t o_lo, uint64_t o_hi,
+                                             uint64_t idx_lo, uint64_t idx_hi) {
+  const uint64_t M0l = P.M0_lo, M0h = P.M0_hi;
+  const uint32_t q0 = P.q0;
+  // Superstrip base in parallel: lane j < nb takes the digit of ring j (w / prefix_j mod r_j)
+  // and its term in (h, l) form; a butterfly over the lanes then sums the terms mod M, so every
+  // lane ends with (hu, l) of the base.
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nb = P.nb;
+  uint32_t hu = 0;
+  uint64_t ll = 0, lh = 0;
+  if (lane < nb) {
+    const uint64_t qv = w / P.base_prefix[lane];
+    const uint32_t d = lane + 1 == nb ? (uint32_t)qv : (uint32_t)(qv % P.base_radix[lane]);
+    const uint32_t o = P.base_off[lane] + d;
+    hu = P.term_h[o];
+    ll = P.term_l[2 * o];
+    lh = P.term_l[2 * o + 1];
+  }
+  for (uint32_t sh = 1; sh < nb; sh <<= 1) {
+    const uint32_t src = lane ^ sh;
+    const uint32_t oh = es_from_lane(src, hu);
+    const uint64_t ol = (uint64_t)es_from_lane(src, (uint32_t)ll) |
+                        ((uint64_t)es_from_lane(src, (uint32_t)(ll >> 32)) << 32);
+    const uint64_t olh = (uint64_t)es_from_lane(src, (uint32_t)lh) |
+                         ((uint64_t)es_from_lane(src, (uint32_t)(lh >> 32)) << 32);
+    const uint64_t nl = ll + ol;
+    lh = lh + olh + (nl < ll ? 1ull : 0ull);
+    ll = nl;
+    uint32_t nh = hu + oh;
+    if (es_ge128(lh, ll, M0h, M0l)) {
+      const uint64_t bl = ll < M0l ? 1ull : 0ull;
+      ll -= M0l;
+      lh = lh - M0h - bl;
+      nh += 1u;
+    }
+    if (nh >= q0) nh -= q0;
+    if (nh >= q0) nh -= q0;
+    hu = nh;
+  }
+  // lane 0's block holds the total: make it wave-uniform
+  hu = (uint32_t)__builtin_amdgcn_readlane((int)hu, 0);
+  ll = es_readlane64(ll, 0);
+  lh = es_readlane64(lh, 0);
+  // row of the base's phase (block constant included) in table f, computed by lane f
+  cu32_t* mk = (cu32_t*)(P.masks);
+  uint32_t rowf = 0;
+  if (lane < (uint32_t)F) {
+    const uint32_t* rec = P.masks + kMaskInfo * lane;
+    const uint32_t m = rec[0];
+    const uint64_t bm = (uint64_t)hu * rec[2] + es_mod128(ll, lh, m, rec[3], rec[4], rec[5]) +
+                        rec[7];
+    rowf = (uint32_t)((uint64_t)(uint32_t)(bm % m) * rec[6] % m);
+  }
+  const uint32_t pe = P.perm[(size_t)g * 64u + lane];
+  const uint32_t sg = pe & 0x7fffffffu;
+  const bool valid = (pe >> 31) == 0;
+  cu32_t* srow = (cu32_t*)P.strip + (size_t)sg * P.stride;
+  const uint64_t Ll = (uint64_t)srow[2] | ((uint64_t)srow[3] << 32);
+  const uint64_t Lh = (uint64_t)srow[4] | ((uint64_t)srow[5] << 32);
+  // Lp = L_sg + l_w (mod M0), Hp = H_sg + h_w (+1 on the carry), then reduced below q0: a
+  // lane whose strip row plus superstrip passes M starts 32 rows on in every table
+  uint64_t Lpl = Ll + ll;
+  uint64_t Lph = Lh + lh + (Lpl < Ll ? 1ull : 0ull);
+  uint32_t Hp = srow[0] + hu;
+  if (es_ge128(Lph, Lpl, M0h, M0l)) {
+    const uint64_t bl = Lpl < M0l ? 1ull : 0ull;
+    Lpl -= M0l;
+    Lph = Lph - M0h - bl;
+    Hp += 1u;
+  }
+  const uint32_t lane_wrap = Hp >= q0 ? 256u : 0u;
+  const uint32_t lane_wrap_last = Hp >= q0 ? (P.last_even ? 8u : 256u) : 0u;
+  if (Hp >= q0) Hp -= q0;
+  uint32_t A[F + 1];
+#pragma unroll
+  for (int f = 0; f < F; ++f)
+    A[f] = mk[kMaskInfo * f + 1] + srow[8 + f] +
+           8u * (uint32_t)__builtin_amdgcn_readlane((int)rowf, f) +
+           (f == F - 1 ? lane_wrap_last : lane_wrap);
+  A[F] = 0;
+  const uint64_t span = (uint64_t)P.V * P.S;
+  const uint64_t obase = w * span;
+  const bool full = obase >= o_lo && obase + span <= o_hi;
+  if (full)
+    es_rows_sweep<F, C, false>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo, o_hi,
+                               idx_lo, idx_hi);
+  else
+    es_rows_sweep<F, C, true>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo, o_hi,
+                              idx_lo, idx_hi);
+}
+
+// Units g + G * (w - w_lo) for w in [w_lo, w_hi), one per wave, strided over the grid.
+template <int F, int C>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uint64_t o_lo,
+               uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
+  const RowPlan& P = *Pp;
+  extern __shared__ uint32_t lds[];
+  for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
+  __syncthreads();
+  const uint32_t wpg = blockDim.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t units = (uint64_t)P.G * (w_hi - w_lo);
+  const uint64_t step = (uint64_t)gridDim.x * wpg;
+  for (uint64_t q = (uint64_t)blockIdx.x * wpg + wave; q < units; q += step) {
+    const uint64_t wq = q / P.G;
+    es_rows_unit<F, C>(P, lds, (uint32_t)(q - wq * P.G), w_lo + wq, o_lo, o_hi, idx_lo, idx_hi);
+  }
+}
+
+// Grid-mapped batch of row-lane jobs: workgroup b serves units (b - wg_start) * waves + wave
+// of the job whose workgroup range holds b.  GridJob's u_lo/u_hi hold the job's superstrips.
+template <int F, int C>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
+                     uint32_t n_jobs) {
+  extern __shared__ uint32_t lds[];
+  const uint64_t b = blockIdx.x;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].wg_start <= b) lo = mid; else hi = mid;
+  }
+  const GridJob J = jobs[lo];
+  const RowPlan P = plans[J.plan];
+  for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
+  __syncthreads();
+  const uint32_t wpg = blockDim.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t q = (b - J.wg_start) * wpg + wave;
+  if (q < (uint64_t)P.G * (J.u_hi - J.u_lo)) {
+    const uint64_t wq = q / P.G;
+    es_rows_unit<F, C>(P, lds, (uint32_t)(q - wq * P.G), J.u_lo + wq, J.o_lo, J.o_hi, J.idx_lo,
+                       J.idx_hi);
+  }
+}
+
+typedef void (*RowsFn)(const RowPlan*, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
+typedef void (*RowsBatchFn)(const RowPlan*, const GridJob*, uint32_t);
+
+template <int F>
+RowsFn rows_kernel_for(uint32_t C) {
+  if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  switch (C) {
+    case 0: return es_rows_kernel<F, 0>;
+    case 1: return es_rows_kernel<F, 1>;
+    case 2: return es_rows_kernel<F, 2>;
+    case 3: return es_rows_kernel<F, 3>;
+    default: return nullptr;
+  }
+}
+
+template <int F>
+RowsBatchFn rows_batch_for(uint32_t C) {
+  if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  switch (C) {
+    case 0: return es_rows_batch_kernel<F, 0>;
+    case 1: return es_rows_batch_kernel<F, 1>;
+    case 2: return es_rows_batch_kernel<F, 2>;
+    case 3: return es_rows_batch_kernel<F, 3>;
+    default: return nullptr;
+  }
+}
+
 typedef void (*KernelFn)(HotPlan, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
 typedef void (*GridBatchFn)(const HotPlan*, const GridJob*, uint32_t);
 
@@ -527,6 +865,14 @@ GridBatchFn pick_grid_batch_p0(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p1(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p2(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p3(int F, uint32_t W, uint32_t G, uint32_t C);
+RowsFn pick_rows_p0(int F, uint32_t C);
+RowsFn pick_rows_p1(int F, uint32_t C);
+RowsFn pick_rows_p2(int F, uint32_t C);
+RowsFn pick_rows_p3(int F, uint32_t C);
+RowsBatchFn pick_rows_batch_p0(int F, uint32_t C);
+RowsBatchFn pick_rows_batch_p1(int F, uint32_t C);
+RowsBatchFn pick_rows_batch_p2(int F, uint32_t C);
+RowsBatchFn pick_rows_batch_p3(int F, uint32_t C);
 
 // Mask counts served by each instance unit.
 constexpr int kPartFirstF[5] = {0, 4, 7, 10, 13};

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libes_sieve.so")
 
-ES_ABI_VERSION = 2
+ES_ABI_VERSION = 3
 ES_OK, ES_ERR_INVALID, ES_ERR_DEVICE, ES_ERR_OVERFLOW, ES_ERR_LIMIT = 0, -1, -2, -3, -4
 
 u32p = C.POINTER(C.c_uint32)
@@ -30,7 +30,7 @@ class PlanDesc(C.Structure):
         ("mask_bits", u32p), ("n_mask_words", C.c_uint32),
         ("strip_target", C.c_uint32), ("threads", C.c_uint32), ("out_cap", C.c_uint64),
         ("lds_budget", C.c_uint32), ("global_masks", C.c_uint32), ("launch_log2", C.c_uint32),
-        ("inner_mod", C.c_uint32), ("lead_radix", C.c_uint32),
+        ("inner_mod", C.c_uint32), ("lead_radix", C.c_uint32), ("layout", C.c_uint32),
     ]
 
 
@@ -40,7 +40,8 @@ class EngineInfo(C.Structure):
         ("strips", C.c_uint64), ("r0", C.c_uint32), ("q0", C.c_uint32),
         ("strip_len", C.c_uint32), ("n_masks_used", C.c_uint32), ("lds_bytes", C.c_uint32),
         ("words", C.c_uint32), ("threads", C.c_uint32), ("device", C.c_uint32),
-        ("global_masks", C.c_uint32), ("reserved", C.c_uint32),
+        ("global_masks", C.c_uint32), ("layout", C.c_uint32), ("steps", C.c_uint32),
+        ("groups", C.c_uint32),
     ]
 
 

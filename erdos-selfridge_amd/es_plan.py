@@ -51,6 +51,8 @@ TUNABLES = {
     "mi_masks": "auto",        # word-mask prefilters staged in LDS ("auto": by candidate rate)
     "mi_lds_budget": 65536,    # bytes of LDS tables per workgroup
     "mi_global_masks": 0,      # masks read through L1 instead of LDS (0, 2, 4; r0 <= 32)
+    "mi_layout": "auto",       # "auto" | "strips" | "rows": lanes over strips, or over strip
+                               # rows grouped by LDS bank ("auto": rows where they apply)
     "mi_inner_radix": "auto",  # ring 0 mod p^T wider than this is swept as digit rings (0:
                                # never; "auto": the engine times the distinct splits)
     "mi_ring_swap": "auto",    # "auto": a ring 0 of 33..128 residues trades places with a

@@ -67,6 +67,7 @@ class PlanSpace:
         # word-mask pass rate of p as a filter outside the wheel: the deepest digit sub-filter
         # mod p^D <= 4096 (1 when there is none; a wheel prime divides M and gets no mask)
         self.mrate = np.ones_like(self.cnt)
+        self.mmod = np.zeros_like(self.cnt)                       # that sub-filter's modulus
         for i, p in enumerate(self.ps):
             for T in range(0, min(1, int(self.tp[i]))):
                 D = T
@@ -74,6 +75,7 @@ class PlanSpace:
                     D += 1
                 if D > T:
                     self.mrate[i, T] = self.cnt_at(i, D) / p ** D / (self.cnt[i, T] / p ** T)
+                    self.mmod[i, T] = p ** D
         dens = float(np.prod(self.rate[:, 0]))
         self.log2E = -math.log2(dens)
         self.small = min(N_SMALL, P)
@@ -200,11 +202,13 @@ class PlanSpace:
           0-15   rings 0..7 in engine order (radix descending, then p): log2 q, log2 radix;
           16-23  kernel geometry: inner radix (ring 0, or its lowest digit group when wider
                  than 128), words per row, split flag, strip length S (leading radices nearest
-                 1024), rings decoded per strip, log2 of the exact-tail fraction left by the 8
-                 best word masks, ring count, log2 of a per-position cost estimate of the
-                 kernel (its LDS row reads, W x (masks + 1) per outer index, with the mask
-                 count the engine's "auto" rule takes, plus the queued candidates the masks
-                 leave, over the inner radix);
+                 1024), steps per unit V (row-lane layout) or rings decoded per strip
+                 (strip-lane), log2 of the exact-tail fraction left by the 8 best word masks,
+                 the row-lane flag, log2 of a per-position cost estimate of the kernel (its
+                 LDS row reads per outer index with the mask count the engine's "auto" rule
+                 takes within the LDS budget -- fewer bank cycles and padding lanes in the
+                 row-lane layout -- plus the queued candidates the masks leave, over the inner
+                 radix);
           24-31  exponent fraction T_p / t_p of the 8 smallest primes;
           32-39  the engine's split alternatives for ring 0 (whole, digit groups <= 64, 32,
                  16): log2 inner radix and words per row of each (the engine times them and
@@ -258,21 +262,43 @@ class PlanSpace:
         kr = np.where(split[:, None], kr_split, np.where(swap[:, None], kr_swap, kr_whole))
         S, m = self._strips(kr, mmin=np.where(swap, jl, 0))
         nk = (kr > 0).sum(1)
-        mr = np.sort(self._take(self.mrate, E), axis=1)[:, :N_LEAD]
+        # steps per unit of the row-lane layout: the radices after the strip rings up to 1024
+        V = np.ones(n)
+        live = np.ones(n, dtype=bool)
+        for j in range(1, kr.shape[1]):
+            r = kr[:, j]
+            sub = live & (j > m) & (r > 0) & (V * np.maximum(r, 1) <= 1024)
+            live &= ~((j > m) & ~sub)
+            V = np.where(sub, V * r, V)
+        rowl = (W == 1) & (S >= 32) & (V >= 16)        # the engine's row-lane layout
+        mall = self._take(self.mrate, E)
+        mo = np.argsort(mall, axis=1, kind="stable")[:, :N_LEAD]
+        mr = np.take_along_axis(mall, mo, 1)
+        mq = np.take_along_axis(self._take(self.mmod, E), mo, 1)
         tail = np.log2(mr).sum(1) + np.log2(r0k)
         F[:, o + 0] = np.log2(r0k) / 8.0
         F[:, o + 1] = W / 4.0
         F[:, o + 2] = split
         F[:, o + 3] = np.log2(S) / 16.0
-        F[:, o + 4] = np.maximum(nk - m, 0) / 16.0
+        F[:, o + 4] = np.where(rowl, np.log2(V) / 16.0, np.maximum(nk - m, 0) / 16.0)
         F[:, o + 5] = tail / -32.0
-        F[:, o + 6] = nr / 20.0
+        F[:, o + 6] = rowl
         # masks the "auto" rule stages: least pass rates while r0 x the rates taken >= the stop
+        # and their tables fit the LDS budget (row-lane tables: rows [0, 3m + 64) of 8 bytes;
+        # strip-lane: [0, 2m) of 8 W bytes)
         pre = r0k[:, None] * np.concatenate([np.ones((n, 1)), np.cumprod(mr, 1)[:, :-1]], 1)
-        take = (pre >= AUTO_MASK_STOP) & (mr < 1.0)
+        tbytes = np.where(rowl[:, None], 24.0 * mq + 528.0, 16.0 * W[:, None] * mq + 16.0)
+        fits = np.cumsum(np.where(mr < 1.0, tbytes, 0.0), 1) <= 65536.0 - 4.0 * W[:, None] * 512
+        take = (pre >= AUTO_MASK_STOP) & (mr < 1.0) & fits
         nm = take.sum(1)
         left = r0k * np.where(take, mr, 1.0).prod(1)   # candidate slots per outer index
-        F[:, o + 7] = np.log2((W * (nm + 1.0) + 20.0 * left) / r0k) / 8.0
+        # per-position cost estimate: LDS row reads (row-lane: about two thirds of the
+        # strip-lane bank cycles, padding lanes included, plus the unit set-up over V steps)
+        # and the queued candidates the masks leave
+        pad = np.ceil(S / 64.0) * 64.0 / S
+        cost_s = W * (nm + 1.0) + 20.0 * left
+        cost_r = (0.67 * (nm + 1.0) + 20.0 * left) * pad + 8.0 / V
+        F[:, o + 7] = np.log2(np.where(rowl, cost_r, cost_s) / r0k) / 8.0
         o += 8
         F[:, o:o + self.small] = Ei[:, :self.small] / self.tp[None, :self.small]
         o += N_SMALL

@@ -33,6 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 MASK_MOD_CAP = 4096          # largest prefilter modulus staged in LDS
 MAX_MASKS = 12
+ROWS_MAX_MASKS = 11          # row-lane kernel instances exist for 1..11 masks
 AUTO_MASK_STOP = 0.01        # "auto" masks: stop below this many candidate slots per outer index
 
 
@@ -126,9 +127,43 @@ def _table_words(k: int, p: int, T: int) -> np.ndarray:
     return w
 
 
+def engine_geometry(radix, lead_rings: int = 0, strip_target: int = 1024):
+    """Strip length S, strip ring count m and steps per unit V the engine derives from the
+    kernel's ring radices (es_engine_create): strip rings 1..m are the leading product nearest
+    the target in log scale (a lead block is always whole), the sub rings after them the
+    product up to 1024."""
+    W = (radix[0] + 31) // 32 if radix else 1
+    S, m = 1, 0
+    while 1 + m < len(radix):
+        nxt = S * radix[1 + m]
+        if nxt > (1 << 22) or nxt * W >= (1 << 26):
+            break
+        if m > 0 and m >= lead_rings and nxt > strip_target:
+            if nxt * S < strip_target * strip_target:
+                S, m = nxt, m + 1
+            break
+        S, m = nxt, m + 1
+    V, j = 1, 1 + m
+    while j < len(radix) and V * radix[j] <= 1024:
+        V *= radix[j]
+        j += 1
+    return S, m, V
+
+
+def rows_layout(radix, lead_rings: int, tun: dict) -> bool:
+    """Whether the engine sweeps this ring order in the row-lane layout (one-word rows, every
+    mask in LDS, a strip of >= 32 rows and >= 16 steps, not forced to strips)."""
+    if str(tun.get("mi_layout", "auto")) == "strips" or int(tun.get("mi_global_masks", 0)):
+        return False
+    if not radix or radix[0] > 32:
+        return False
+    S, _, V = engine_geometry(radix, lead_rings, int(tun.get("mi_strip_target", 1024)))
+    return S >= 32 and V >= 16
+
+
 def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
                  lds_budget: int = 65536, r0: int | None = None, q0: int | None = None,
-                 stop: float = 0.0):
+                 stop: float = 0.0, rows: bool = False):
     """Word-mask prefilters: low-digit conditions mod p^D (D <= t_p, p^D <= MASK_MOD_CAP) of the
     filter primes outside the wheel -- each a necessary condition of its full test (a wheel
     prime divides M, and the engine's mask tables need M invertible mod p^D).  Candidates are taken
@@ -136,7 +171,8 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     twice over, two words per ring-0 word per row) fit the budget beside the ring-0 carry table.
     With `stop` > 0, masks are added only while the expected candidate slots per outer index
     (r0 x the pass rates taken) is at least `stop`: below that, a further mask costs more LDS
-    reads than the queued exact re-test it saves.
+    reads than the queued exact re-test it saves.  rows: the row-lane layout's tables, rows
+    [0, 3m + 64) of one 8-byte row per ring-0 word, instead of [0, 2m) of two words.
     Returns [(filter_index, D, modulus, conditional pass rate)]."""
     k = plan.k
     rest = plan.rest if limit is None else plan.rest[:limit]
@@ -165,7 +201,7 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     for rate, fi, d, m in cands:
         if len(out) >= n_masks or dens < stop:
             break
-        cost = 4 * words * m + 4      # rows [0, 2m), two words per ring-0 word
+        cost = (2 * (3 * m + 64) + 4) if rows else (4 * words * m + 4)
         if fi in taken or cost > left:
             continue
         taken.add(fi)
@@ -236,9 +272,17 @@ class _Desc:
         self.filt_bits = np.concatenate(words) if words else np.zeros(1, dtype=np.uint32)
         nm = tun.get("mi_masks", "auto")
         auto = nm == "auto"
-        self.masks = choose_masks(plan, MAX_MASKS if auto else min(MAX_MASKS, int(nm)), limit,
+        lead_rings = 0
+        if self.lead:
+            prod_ = 1
+            while prod_ < self.lead:
+                lead_rings += 1
+                prod_ *= radix[lead_rings]
+        self.rows = rows_layout(radix, lead_rings, tun)
+        self.masks = choose_masks(plan, (ROWS_MAX_MASKS if self.rows else MAX_MASKS) if auto
+                                  else min(MAX_MASKS, int(nm)), limit,
                                   int(tun.get("mi_lds_budget", 65536)), self.r0, self.q0,
-                                  AUTO_MASK_STOP if auto else 0.0)
+                                  AUTO_MASK_STOP if auto else 0.0, rows=self.rows)
         self.slow = slow_order(plan, self.masks, limit)
         mm, ms, mo, mw = [], [], [], []
         nw = 0
@@ -287,6 +331,7 @@ class _Desc:
         d.launch_log2 = int(tun.get("mi_launch_log2", 37))
         d.inner_mod = self.inner_mod
         d.lead_radix = self.lead
+        d.layout = {"auto": 0, "strips": 1, "rows": 2}[str(tun.get("mi_layout", "auto"))]
         self.desc = d
 
 
@@ -408,7 +453,8 @@ class GpuSieve:
         assert self.R_outer * self._f == info.R_outer and self.R_outer * self.r0 == P.R
         self.ochunk = max(1, self.chunk // self.r0)
         self.strip_len = info.strip_len
-        self.mask_mode = f"mi355x-mask{info.n_masks_used}-l1{info.global_masks}"
+        self.layout = "rows" if info.layout == 2 else "strips"
+        self.mask_mode = f"mi355x-{self.layout}-mask{info.n_masks_used}-l1{info.global_masks}"
         self.masks = self._d.masks
         self.launches = 0
         self.residues_checked = 0

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ES_ABI_VERSION 2u
+#define ES_ABI_VERSION 3u
 
 typedef enum es_status {
   ES_OK = 0,
@@ -92,6 +92,10 @@ typedef struct es_plan_desc {
   uint32_t inner_mod;          /* modulus of the ring ring 0 is a digit group of (0 = ring_mod[0]) */
   uint32_t lead_radix;         /* 0 or 1: rings in the reference order; > 1: the product of
                                   ring_radix[1..j] of a lead block (see above) */
+  uint32_t layout;             /* 0: row-lane layout where it applies (one-word rows, every
+                                  mask in LDS, at most 11 masks), else strip-lane; 1: always
+                                  strip-lane; 2: row-lane or ES_ERR_LIMIT.  Speed only: both
+                                  layouts return the same survivors. */
 } es_plan_desc;
 
 typedef struct es_engine_info {
@@ -106,7 +110,10 @@ typedef struct es_engine_info {
   uint32_t threads;
   uint32_t device;
   uint32_t global_masks;       /* masks read through L1 */
-  uint32_t reserved;
+  uint32_t layout;             /* 1: strip-lane (a lane walks a strip), 2: row-lane (a lane
+                                  holds a strip row, a wave walks steps of the next rings) */
+  uint32_t steps;              /* row-lane: steps per unit (V); strip-lane: 0 */
+  uint32_t groups;             /* row-lane: lane groups of 64 strip rows (G); strip-lane: 0 */
 } es_engine_info;
 
 /* Last error message of the calling thread ("" if none).  Never NULL. */
@@ -160,7 +167,7 @@ int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t
  * result[4] = {count (may exceed cap), least residue s lo, hi (n = tM + s > k+1), 1 if there
  * is a minimum else 0} to the caller's device memory.  Asynchronous; allocates nothing: its
  * scratch (counters, the minimum's slot, one 16-byte residue record per survivor slot, the
- * exact test's plan record) is a file-scope device array of 1 MiB + 512 bytes that every GPU
+ * exact test's plan records) is a file-scope device array of 1 MiB + 1 KiB that every GPU
  * holds until the process ends.  cap above ES_STATIC_MAX_IDS fails with ES_ERR_LIMIT.  Calls
  * that overlap on one GPU race on the shared scratch.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (HIP graph capture, tight serving loops). */

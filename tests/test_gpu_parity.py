@@ -207,8 +207,8 @@ def test_lds_short_masks_stay_exact(monkeypatch):
     import gpu_sieve_mi355x as G
     orig = G.choose_masks
     monkeypatch.setattr(G, "choose_masks",
-                        lambda plan, n, limit=None, lds_budget=0, r0=None, q0=None, stop=0.0:
-                        orig(plan, n, limit, 1 << 30, r0, q0, stop))
+                        lambda plan, n, limit=None, lds_budget=0, r0=None, q0=None, stop=0.0,
+                        rows=False: orig(plan, n, limit, 1 << 30, r0, q0, stop, rows=rows))
     for k, limit in ((100, 5), (101, None)):
         plan = EP.Plan(k)
         span = min(1 << 21, plan.R)
@@ -382,3 +382,40 @@ def test_block_overflow_falls_back_to_per_launch_drain():
     assert n == 12654618 > eng.out_cap
     got = np.sort(np.asarray(ids, dtype=np.uint64))
     assert len(got) == n and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("k,wheel", [(30, C1_WHEEL), (200, "fullpower"), (304, None), (309, None),
+                                     (318, None), (320, None), (378, None), (399, K399_PRIMARY),
+                                     (399, K399_CONFIRM)])
+def test_row_lane_layout_equals_strip_lane(k, wheel):
+    """The row-lane layout (lanes over grouped strip rows, bank-neutral row map) and the
+    strip-lane layout return the same survivors and minimum on the same windows: dense
+    filter-prefix windows at unaligned edges and several launches, and the oracle agrees.
+    k=304 and 320 stage a mask mod 2^9 (2 is not in their wheels): the power-of-2 table keeps
+    the one-row wrap map."""
+    import gpu_sieve_mi355x as G
+    if wheel == "fullpower":
+        wheel = [list(x) for x in EP.fullpower_wheel(200)]
+    rng = random.Random(k)
+    plan = EP.Plan(k, None, wheel)
+    for limit in (3, 6, None):
+        tune = {"max_survivors": 1 << 22, "mi_launch_log2": 26}
+        if limit is not None:
+            tune["mi_filter_limit"] = limit
+        rows = G.GpuSieve(EP.Plan(k, dict(tune, mi_layout="auto"), wheel))
+        strips = G.GpuSieve(EP.Plan(k, dict(tune, mi_layout="strips"), wheel))
+        assert strips.layout == "strips"
+        span = min(1 << 24, plan.R // 3)
+        t = rng.randrange(0, 1 << 16)
+        lo = rng.randrange(0, plan.R - span)
+        hi = lo + span - rng.randrange(0, 1000)
+        say(f"case layouts k={k} limit={limit} t={t} window=[{lo},{hi}) layout={rows.layout} "
+            f"steps={rows.info.steps} groups={rows.info.groups} strip={rows.strip_len}")
+        a, amin = rows.scan_window(t, lo, hi)
+        b, bmin = strips.scan_window(t, lo, hi)
+        assert sorted(a) == sorted(b) and amin == bmin
+        filt = plan.rest if limit is None else plan.rest[:limit]
+        n, want, wmin = O.c_scan_min(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
+        assert sorted(a) == [int(x) for x in want] and amin == wmin
+    if rows.info.words == 1 and rows.strip_len >= 32:
+        assert rows.layout == "rows"

@@ -427,9 +427,10 @@ def bench_valu(a) -> int:
 
 
 def bench_stream(a) -> int:
-    """Streaming rates of library kernels on 16 GiB (torch's sum: read only; device copy:
-    read + write), HIP events, best of `steps` -- the practical HBM ceiling the filter pass is
-    compared with."""
+    """Streaming rates on 16 GiB: library kernels (torch's sum: read only; device copy: read +
+    write) and the hand-written dwordx4 read kernel over launch shapes (blocks, threads, loads
+    in flight, nontemporal or not); HIP events, best of `steps` -- the HBM read ceiling the
+    filter pass is compared with."""
     import torch
     n = 16 * (1 << 30) // 4
     x = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -447,6 +448,24 @@ def bench_stream(a) -> int:
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1))
         out[name] = {"ms": best, "GB_per_s": nbytes / (best * 1e-3) / 1e9}
+    # the hand-written dwordx4 streaming read (es_calib_hbm_read), over launch shapes
+    import ctypes as C
+    import es_native as nat
+    sweep = []
+    for nt in (0, 1):
+        for unroll in (1, 4, 8):
+            for threads in (256, 512):
+                for blocks in (1024, 2048, 4096, 8192, 16384):
+                    ms = C.c_double()
+                    nat.check(nat.lib().es_calib_hbm_read(
+                        C.c_void_p(x.data_ptr()), 4 * n, blocks, threads, unroll, nt,
+                        max(1, a.steps), C.byref(ms)))
+                    sweep.append({"nontemporal": nt, "unroll": unroll, "threads": threads,
+                                  "blocks": blocks, "ms": ms.value,
+                                  "GB_per_s": 4 * n / (ms.value * 1e-3) / 1e9})
+    best = max(sweep, key=lambda r: r["GB_per_s"])
+    out["dwordx4_read"] = {"best": best, "frac_of_peak": best["GB_per_s"] * 1e9 / HBM_PEAK,
+                           "sweep": sweep}
     print(json.dumps({"workload": "stream", "bytes": 4 * n, "peak_GB_per_s": HBM_PEAK / 1e9,
                       **out}), flush=True)
     return 0
@@ -470,7 +489,9 @@ def bench_batch(a) -> int:
         space = dict(es_autotune.SPACE, mi_global_masks=[0])
         for k in range(300, 321):
             tuned[k], _ = es_autotune.autotune(k, space=space, probe_log2=34, method="coord")
-    engines = {k: G.GpuSieve(es_plan.Plan(k, tuned.get(k))) for k in range(300, 321)}
+    lay = {"mi_layout": a.layout} if a.layout is not None else {}
+    engines = {k: G.GpuSieve(es_plan.Plan(k, {**tuned.get(k, {}), **lay}))
+               for k in range(300, 321)}
     jobs = []
     for k, eng in engines.items():
         for _ in range(4):
@@ -535,6 +556,7 @@ def bench_batch(a) -> int:
         "vs_baseline": None, "dtype": "u128/u32 integer", "data": "synthetic seeded windows",
         "config": {"workload": "21 plans x 4 blocks x 2^34 positions", "jobs": len(jobs),
                    "launches": launches, "autotuned": bool(a.autotune),
+                   "row_lane_plans": sum(e.layout == "rows" for e in engines.values()),
                    "tuned": {str(k): v for k, v in tuned.items()}},
         "roofline": batch_roofline(engines, jobs, N, pos / (bms / 1e3), bms, launches),
         "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
@@ -572,6 +594,8 @@ def bench_k200(a) -> int:
     k, N = 200, 1 << 36
     wheel = es_plan.fullpower_wheel(k)
     tune = {"max_survivors": 1 << 20}
+    if a.layout is not None:
+        tune["mi_layout"] = a.layout
     if a.autotune:
         import es_autotune
         best, _ = es_autotune.autotune(k, wheel, probe_log2=36, method="coord")

@@ -96,6 +96,32 @@ __global__ void __launch_bounds__(1024) es_lds_read_kernel(uint32_t iters, uint3
   if (acc == 0x9e3779b9u) sink[blockIdx.x] = acc;  // keeps the reads live
 }
 
+// HBM streaming read: each thread reads 16-byte vectors (global_load_dwordx4) at
+// grid-stride positions, U of them in flight before it folds them into its checksum; NT marks
+// the loads nontemporal (streamed past the caches).  A launch reads every byte once.
+typedef uint32_t es_v4u __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(512) es_hbm_read_kernel(const es_v4u* __restrict__ src,
+                                                          uint64_t n16,
+                                                          uint32_t* __restrict__ sink) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  es_v4u acc = {0u, 0u, 0u, 0u};
+  for (; i + (uint64_t)(U - 1) * stride < n16; i += (uint64_t)U * stride) {
+    es_v4u v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      v[u] = NT ? __builtin_nontemporal_load(src + i + (uint64_t)u * stride)
+                : src[i + (uint64_t)u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc ^= v[u];
+  }
+  for (; i < n16; i += stride) acc ^= src[i];
+  const uint32_t x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (x == 0x9e3779b9u) sink[threadIdx.x] = x;  // keeps the loads live
+}
+
 }  // namespace
 
 namespace {
@@ -163,5 +189,47 @@ extern "C" int es_calib_lds(uint32_t blocks, uint32_t iters, uint32_t mode, doub
   if (e1) (void)hipEventDestroy(e1);
   if (sink) (void)hipFree(sink);
   if (rc == ES_OK) *bytes = (uint64_t)blocks * 1024u * iters * 8u * 8u;
+  return rc;
+}
+
+extern "C" int es_calib_hbm_read(const void* src, uint64_t bytes, uint32_t blocks,
+                                 uint32_t threads, uint32_t unroll, uint32_t nontemporal,
+                                 uint32_t reps, double* ms) {
+  if (!src || !ms || bytes < 16 || bytes % 16 || blocks == 0 || reps == 0 ||
+      (threads != 256 && threads != 512) || ((uintptr_t)src & 15u))
+    return es::fail(ES_ERR_INVALID, "es_calib_hbm_read: bad argument");
+  using Fn = void (*)(const es_v4u*, uint64_t, uint32_t*);
+  Fn fn = nullptr;
+  if (unroll == 1) fn = nontemporal ? es_hbm_read_kernel<1, true> : es_hbm_read_kernel<1, false>;
+  if (unroll == 4) fn = nontemporal ? es_hbm_read_kernel<4, true> : es_hbm_read_kernel<4, false>;
+  if (unroll == 8) fn = nontemporal ? es_hbm_read_kernel<8, true> : es_hbm_read_kernel<8, false>;
+  if (!fn) return es::fail(ES_ERR_INVALID, "es_calib_hbm_read: unroll must be 1, 4 or 8");
+  uint32_t* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = ES_OK;
+  if (hipMalloc(&sink, 512 * 4) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = es::fail(ES_ERR_DEVICE, "es_calib_hbm_read: allocation failed");
+  const es_v4u* p = reinterpret_cast<const es_v4u*>(src);
+  const uint64_t n16 = bytes / 16;
+  if (rc == ES_OK) {
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads), 0, nullptr, p, n16, sink);  // warm-up
+    if (hipGetLastError() != hipSuccess) rc = es::fail(ES_ERR_DEVICE, "es_calib_hbm_read: launch");
+  }
+  double best = 1e30;
+  for (uint32_t r = 0; rc == ES_OK && r < reps; ++r) {
+    float t = 0.f;
+    if (hipEventRecord(e0, nullptr) != hipSuccess) rc = es::fail(ES_ERR_DEVICE, "event");
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads), 0, nullptr, p, n16, sink);
+    if (rc == ES_OK && (hipGetLastError() != hipSuccess || hipEventRecord(e1, nullptr) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = es::fail(ES_ERR_DEVICE, "es_calib_hbm_read: launch");
+    best = std::min(best, (double)t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (sink) (void)hipFree(sink);
+  if (rc == ES_OK) *ms = best;
   return rc;
 }

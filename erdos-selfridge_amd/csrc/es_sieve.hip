@@ -271,16 +271,19 @@ void outer_range(const es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t*
 int launch_kernel(es_engine* e, hipStream_t st, const ColdPlan* cold, const RowPlan* rows_plan,
                   uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
   if (e->rows) {
+    // one wave per unit; superstrips [w_lo, w_hi) in launches of at most max_grid workgroups
     const uint64_t span = (uint64_t)e->S * e->V;
-    uint64_t w_lo = o_lo / span, w_hi = (o_hi + span - 1) / span;
-    const uint64_t units = (uint64_t)e->Gl * (w_hi - w_lo);
+    const uint64_t w_end = (o_hi + span - 1) / span;
     const uint32_t wpg = e->threads / 64;
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((units + wpg - 1) / wpg,
-                                                                   e->max_grid));
+    const uint64_t per = std::max<uint64_t>(1, (uint64_t)e->max_grid * wpg / e->Gl);
     const RowPlan* rp = rows_plan ? rows_plan : e->d_rp;
-    void* args[] = {&rp, &w_lo, &w_hi, &o_lo, &o_hi, &idx_lo, &idx_hi};
-    ES_HIP(hipLaunchKernel((const void*)e->rows_kern, dim3((uint32_t)grid), dim3(e->threads),
-                           args, e->lds_bytes, st));
+    for (uint64_t w_lo = o_lo / span; w_lo < w_end; w_lo += per) {
+      uint64_t a = w_lo, b = std::min(w_end, w_lo + per);
+      const uint64_t grid = ((uint64_t)e->Gl * (b - a) + wpg - 1) / wpg;
+      void* args[] = {&rp, &a, &b, &o_lo, &o_hi, &idx_lo, &idx_hi};
+      ES_HIP(hipLaunchKernel((const void*)e->rows_kern, dim3((uint32_t)grid), dim3(e->threads),
+                             args, e->lds_bytes, st));
+    }
   } else {
     uint64_t u_lo = o_lo / e->S, u_hi = (o_hi + e->S - 1) / e->S;
     const uint64_t grid = std::max<uint64_t>(

@@ -502,6 +502,8 @@ es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restric
 // Q_sg + W_w + 32 c_u + D_v + 32 c_v (strip row, superstrip base, the wrap of their sum at
 // unit set-up, step, the wrap at the step) without reduction, so rows [0, 3m + 64) are
 // stored; only Q_sg and c_u differ between the lanes of a wave, and c_u keeps the bank.
+// (Reducing the lane's row mod m at unit set-up would halve that but moves the lanes whose
+// sum passes m by -m rows, to another bank: 1.5x the conflict cycles at k=399, measured.)
 constexpr int kRowsMaxMasks = 11;   // a step row: h, z (4 dwords), then one row offset per mask
 constexpr int kStepDwords = 16;
 
@@ -719,6 +721,9 @@ __device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* l
     Lph = Lph - M0h - bl;
     Hp += 1u;
   }
+  // the lane's row in table f: strip row + superstrip row (+ 32 rows, or one for an even last
+  // modulus, when M was subtracted), left unreduced: a reduction mod m would move the lanes
+  // whose sum passes m to another bank (m is odd), undoing the grouping of the strip rows
   const uint32_t lane_wrap = Hp >= q0 ? 256u : 0u;
   const uint32_t lane_wrap_last = Hp >= q0 ? (P.last_even ? 8u : 256u) : 0u;
   if (Hp >= q0) Hp -= q0;
@@ -740,7 +745,8 @@ __device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* l
                               idx_lo, idx_hi);
 }
 
-// Units g + G * (w - w_lo) for w in [w_lo, w_hi), one per wave, strided over the grid.
+// Units g + G * (w - w_lo) for w in [w_lo, w_hi), one per wave: the grid covers them all (the
+// host splits a range whose units exceed one grid).
 template <int F, int C>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uint64_t o_lo,
@@ -752,8 +758,8 @@ es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uin
   const uint32_t wpg = blockDim.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t units = (uint64_t)P.G * (w_hi - w_lo);
-  const uint64_t step = (uint64_t)gridDim.x * wpg;
-  for (uint64_t q = (uint64_t)blockIdx.x * wpg + wave; q < units; q += step) {
+  const uint64_t q = (uint64_t)blockIdx.x * wpg + wave;
+  if (q < units) {
     const uint64_t wq = q / P.G;
     es_rows_unit<F, C>(P, lds, (uint32_t)(q - wq * P.G), w_lo + wq, o_lo, o_hi, idx_lo, idx_hi);
   }

@@ -125,12 +125,16 @@ class FilterPass:
     the order given (order="given") or most selective first (order="rate")."""
 
     def __init__(self, k: int, primes: list[int] | None = None, lds_budget: int = 16384,
-                 out_cap: int = 1 << 24, order: str = "rate"):
+                 out_cap: int = 1 << 24, order: str = "rate", tables=None):
+        """tables: (moduli, word offsets, u64 words) of other admissible-residue bitmaps, one
+        per prime in the order given, in place of the Kummer sets of k (the pass tests
+        membership in whatever sets it is given)."""
         self.k = k
         if primes is None:
             primes = [p for p in primes_upto(k) if p >= 101]
-        self.primes = list(primes) if order == "given" else filter_order(k, primes)
-        qs, offs, words = filter_tables(k, self.primes)
+        self.primes = list(primes) if order == "given" or tables is not None \
+            else filter_order(k, primes)
+        qs, offs, words = tables if tables is not None else filter_tables(k, self.primes)
         self.qs = qs
         self._keep = (np.asarray(qs, dtype=np.uint32), np.asarray(offs, dtype=np.uint32), words)
         h = C.c_void_p()

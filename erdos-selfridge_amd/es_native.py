@@ -106,6 +106,9 @@ def lib() -> C.CDLL:
         "es_calib_valu": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), u64p]),
         "es_calib_lds": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
                                    u64p]),
+        "es_calib_hbm_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_double)]),
         "es_mlp_create": (C.c_int, [C.c_uint32, C.POINTER(eng)]),
         "es_mlp_destroy": (C.c_int, [eng]),
         "es_mlp_set_params": (C.c_int, [eng, f32p, C.c_uint32]),

@@ -163,6 +163,60 @@ def test_gpu_pass_shapes_agree_ragged():
 
 
 @pytest.mark.gpu
+def test_gpu_pass_empty_and_tiny_lists():
+    """No IDs (nothing launched), one ID, and 1, 63, 65, 257 IDs into a longer stream: the
+    lanes past the end of the list load a clamped ID and are masked, in every shape."""
+    prim = FP.filter_order(K, PRIMES[:8])
+    fp = FP.FilterPass(K, prim)
+    spec = FP.generator_spec(K, fp.primes, mod_bits=100)
+    vals = _spec_values(spec, 4096, 11)
+    fp.upload(vals, np.zeros(0, dtype=np.uint32))
+    print("case fpass empty list", flush=True)
+    assert fp.run().size == 0
+    rng = np.random.default_rng(12)
+    for n in (1, 63, 65, 257):
+        ids = rng.choice(len(vals), size=n, replace=False).astype(np.uint32)
+        print(f"case fpass tiny n={n}", flush=True)
+        fp.upload(vals, ids)
+        want = np.sort(O.c_filter_values(K, fp.primes, vals, ids))
+        for unroll in (2, 8):
+            fp.set_shape(unroll, 3)
+            assert np.array_equal(fp.run(), want), (n, unroll)
+
+
+@pytest.mark.gpu
+def test_gpu_pass_arbitrary_bitmaps():
+    """The pass tests membership in whatever bitmaps it is given, in LDS or in global memory.
+    Here the fourth filter's bitmap loses one admissible residue and gains one inadmissible one
+    (no longer a Kummer set) and the LDS budget holds only the first table: the pass must
+    return exactly the values whose residues are in every bitmap (membership checked in
+    Python)."""
+    prim = FP.filter_order(K, PRIMES[:8])
+    qs, offs, words = FP.filter_tables(K, prim)
+    words = words.copy()
+    q3, o3 = qs[3], offs[3]
+    bit = lambda r: int(words[o3 + (r >> 6)] >> np.uint64(r & 63)) & 1
+    on = next(r for r in range(q3) if bit(r))
+    off = next(r for r in range(q3) if not bit(r))
+    words[o3 + (on >> 6)] ^= np.uint64(1) << np.uint64(on & 63)
+    words[o3 + (off >> 6)] ^= np.uint64(1) << np.uint64(off & 63)
+    fp = FP.FilterPass(K, prim, lds_budget=(qs[0] + 63) // 64 * 8, tables=(qs, offs, words))
+    spec = FP.generator_spec(K, prim, mod_bits=100)
+    vals = _spec_values(spec, 1 << 14, 13)
+    # values that hit the changed residues of filter 3, so both edits matter
+    vals += [on + q3 * i for i in range(1, 200)] + [off + q3 * i for i in range(1, 200)]
+    ids = np.random.default_rng(14).permutation(len(vals)).astype(np.uint32)
+    print(f"case fpass forms n={len(vals)} q3={q3}", flush=True)
+    fp.upload(vals, ids)
+
+    def member(v):
+        return all((int(words[o + ((v % q) >> 6)]) >> ((v % q) & 63)) & 1 for q, o in zip(qs, offs))
+    want = np.sort(np.asarray([i for i in ids if member(vals[i])], dtype=np.uint32))
+    assert len(want) > 100
+    assert np.array_equal(fp.run(), want)
+
+
+@pytest.mark.gpu
 def test_gpu_pass_config4_full_size():
     """Config 4 at its full size: 2^30 seeded candidates of the conditioned u128 stream, all 53
     filters in the conditional order the bench uses; every survivor ID equals the oracle's

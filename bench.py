@@ -166,7 +166,8 @@ def main() -> int:
         "config": {"workload": f"k=399 primary wheel (runs.json), block {BLOCK}, "
                                f"2^{a.window_log2} positions per step, all 75 filters",
                    "k": K, "r0": eng.r0, "R": plan.R, "window": N, "masks": F,
-                   "strip": eng.strip_len, "threads": eng.threads},
+                   "strip": eng.strip_len, "threads": eng.threads, "layout": eng.layout,
+                   "steps_per_unit": eng.info.steps if eng.layout == "rows" else None},
         "roofline": roofline, "cpu_baseline": cpu, "time_to_first_block": part2,
     }), flush=True)
     return 0
@@ -233,11 +234,15 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
 # LDS read peak of gfx950 at the nominal clock: 256 CUs x 256 bytes per CU-clock (64 banks of
 # 4 bytes) x 2.4 GHz.  `--workload lds` measures the conflict-free ds_read_b64 rate live.
 LDS_PEAK_NOMINAL = 256 * 256 * 2.4e9
-# Profiled figures of es_sieve_kernel<6, 0, 1, 1> on the k=399 primary wheel (rocprofv3, 2^36-
-# position launches; the counters are per wave-step = one outer index of 64 lanes = 1856
-# positions).  Other instances are not profiled.
-K399_PROFILE = {"sq_lds_idx_active_per_wave_step": 39.7, "sq_lds_bank_conflict_per_wave_step": 24.0,
-                "sq_insts_valu_per_wave_step": 33.6, "hbm_bytes_per_launch": 1.47e6}
+# Profiled figures on the k=399 primary wheel (rocprofv3, 2^36-position launches; the counters
+# are per wave-step = 64 lanes x one ring-0 word = 1856 positions; HBM bytes = FETCH_SIZE x 2.0
+# + WRITE_SIZE, the copy calibration): es_rows_kernel<6, 1> (row-lane layout, the default) and
+# es_sieve_kernel<6, 0, 1, 1> (strip-lane).  Other instances are not profiled.
+K399_PROFILE = {
+    "rows": {"sq_lds_idx_active_per_wave_step": 31.11, "sq_lds_bank_conflict_per_wave_step": 14.07,
+             "sq_insts_valu_per_wave_step": 33.23, "hbm_bytes_per_launch": 1.446e6},
+    "strips": {"sq_lds_idx_active_per_wave_step": 39.7, "sq_lds_bank_conflict_per_wave_step": 24.0,
+               "sq_insts_valu_per_wave_step": 33.6, "hbm_bytes_per_launch": 1.47e6}}
 
 
 def lds_read_peak(blocks_per_cu: int = 2, iters: int = 8192) -> dict:
@@ -269,21 +274,23 @@ def sieve_roofline(eng, plan, kern_rate: float, dev_ms: float, launches: int, ld
     bpp = eng.lds_bytes_per_position()
     peak = lds["conflict_free_bytes_per_s"]
     achieved = bpp * kern_rate
-    profiled = eng.info.n_masks_used == 6 and eng.info.words == 1 and eng.r0 == 29
+    prof = K399_PROFILE.get(eng.layout) \
+        if eng.info.n_masks_used == 6 and eng.info.words == 1 and eng.r0 == 29 else None
+    profiled = prof is not None
     ef = plan.expected_filter_evals()
     ops_ref = 12 + 10 * ef
     return {
         "bound": "lds", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "GB/s",
         "frac": achieved / peak,
-        "traffic": K399_PROFILE["hbm_bytes_per_launch"] if profiled else None,
+        "traffic": prof["hbm_bytes_per_launch"] if profiled else None,
         "lds_bytes_per_position": bpp,
         "lds_peak_nominal_GBps": LDS_PEAK_NOMINAL / 1e9,
         "lds_random_rows_GBps": lds["random_rows_bytes_per_s"] / 1e9,
         "lds_bytes_per_cu_clock": lds["bytes_per_cu_clock_at_2p4GHz"],
         "lds_cycles_per_wave_step_profiled":
-            K399_PROFILE["sq_lds_idx_active_per_wave_step"] if profiled else None,
+            prof["sq_lds_idx_active_per_wave_step"] if profiled else None,
         "lds_conflict_cycles_per_wave_step_profiled":
-            K399_PROFILE["sq_lds_bank_conflict_per_wave_step"] if profiled else None,
+            prof["sq_lds_bank_conflict_per_wave_step"] if profiled else None,
         "kernel_avg_launch_ms": dev_ms / max(1, launches), "launches": launches,
         "kernel_positions_per_s": kern_rate,
         "reference_algorithm": {

@@ -195,19 +195,91 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     cands.sort(key=lambda c: (c[0], c[1], -c[2]))
     # carry table: q0 rows of `words` words, or q0 u16 suffix starts when words > 2
     carry = (q0 + 1) // 2 if words > 2 else q0 * words
-    left = lds_budget // 4 - carry - 8
-    taken, out = set(), []
+    room = lds_budget // 4 - carry - 8
+
+    def cost_of(m):
+        return (2 * (3 * m + 64) + 4) if rows else (4 * words * m + 4)
+
+    def greedy(left):
+        taken, out = set(), []
+        dens = float(r0)
+        for rate, fi, d, m in cands:
+            if len(out) >= n_masks or dens < stop:
+                break
+            cost = cost_of(m)
+            if fi in taken or cost > left:
+                continue
+            taken.add(fi)
+            left -= cost
+            dens *= rate
+            out.append((fi, d, m, rate))
+        return out
+
+    out = greedy(room)
+    if stop > 0 and out != greedy(1 << 62):
+        # The budget cut the greedy short (a large table taken first can crowd out several
+        # small ones): choose within the budget by the kernel's cost per outer index instead.
+        alt = _masks_within_budget(cands, n_masks, room, cost_of, r0)
+        if _mask_cost(alt, r0) < _mask_cost(out, r0):
+            out = alt
+    return out
+
+
+# Cost of a mask set per outer index, in mask-row reads: one read per mask and the carry read,
+# plus ~160 reads' worth of work per candidate slot queued for the exact re-test (row-lane
+# counters: k=301 with 0.041 slots per outer index runs its wave-steps 1.67x slower than k=304
+# with 0.009, at the same six reads per step).
+_EXACT_PER_SLOT = 160.0
+
+
+def _mask_cost(masks, r0: int) -> float:
     dens = float(r0)
-    for rate, fi, d, m in cands:
-        if len(out) >= n_masks or dens < stop:
-            break
-        cost = (2 * (3 * m + 64) + 4) if rows else (4 * words * m + 4)
-        if fi in taken or cost > left:
-            continue
-        taken.add(fi)
-        left -= cost
+    for _, _, _, rate in masks:
         dens *= rate
+    return len(masks) + 1 + _EXACT_PER_SLOT * dens
+
+
+def _masks_within_budget(cands, n_masks: int, room: int, cost_of, r0: int):
+    """Least-cost mask set (one depth per prime, at most n_masks, LDS words within room): a
+    knapsack over the primes' depth options in units of 64 words, minimising the sum of log
+    pass rates for each (count, units), then the cost of each reachable state."""
+    unit = 64
+    U = max(0, room) // unit
+    by_fi = {}
+    for rate, fi, d, m in cands:
+        by_fi.setdefault(fi, []).append((rate, d, m, -(-cost_of(m) // unit)))
+    inf = float("inf")
+    best = np.full((n_masks + 1, U + 1), inf)
+    best[0, 0] = 0.0
+    choice = []                                         # per prime: argmin option per state
+    for fi, opts in sorted(by_fi.items()):
+        nb = best.copy()
+        pick = np.full(best.shape, -1, dtype=np.int32)
+        for oi, (rate, d, m, cu) in enumerate(opts):
+            if cu > U or rate <= 0.0:
+                continue
+            lr = math.log(rate)
+            cand = np.full(best.shape, inf)
+            cand[1:, cu:] = best[:-1, :U + 1 - cu] + lr
+            better = cand < nb
+            nb[better] = cand[better]
+            pick[better] = oi
+        choice.append((fi, opts, pick))
+        best = nb
+    # cost of every reachable (count, units) state; walk the choices back from the best one
+    cnt = np.arange(n_masks + 1)[:, None]
+    cost = cnt + 1 + _EXACT_PER_SLOT * r0 * np.exp(np.minimum(best, 0.0))
+    cost[~np.isfinite(best)] = inf
+    c, u = np.unravel_index(int(np.argmin(cost)), cost.shape)
+    out = []
+    for fi, opts, pick in reversed(choice):
+        oi = int(pick[c, u])
+        if oi < 0:
+            continue
+        rate, d, m, cu = opts[oi]
         out.append((fi, d, m, rate))
+        c, u = c - 1, u - cu
+    out.sort(key=lambda x: (x[3], x[0], -x[1]))
     return out
 
 

@@ -120,6 +120,62 @@ def test_masks_are_necessary_conditions(k):
     assert covered == set(range(len(plan.rest)))
 
 
+def test_mask_knapsack_is_optimal_on_a_small_instance():
+    """The budgeted mask choice equals exhaustive search: 7 primes with up to 3 depths each,
+    at most 4 masks, a budget that binds; cost = masks + 1 + 160 x r0 x prod(rates)."""
+    import itertools
+    import gpu_sieve_mi355x as G
+    rng = random.Random(3)
+    cands = []
+    for fi in range(7):
+        rate = 1.0
+        for d in range(1, 4):
+            rate *= rng.uniform(0.3, 0.9)
+            cands.append((rate, fi, d, rng.choice([101, 257, 600, 1100, 2300])))
+    cost_of = lambda m: 2 * (3 * m + 64) + 4
+    room, r0, n = 14000, 29, 4
+    got = G._masks_within_budget(cands, n, room, cost_of, r0)
+    unit = 64
+    best = None
+    by_fi = {}
+    for c in cands:
+        by_fi.setdefault(c[1], []).append(c)
+    for pick in itertools.product(*[[None] + v for v in by_fi.values()]):
+        chosen = [c for c in pick if c is not None]
+        units = sum(-(-cost_of(c[3]) // unit) for c in chosen)
+        if len(chosen) > n or units > room // unit:
+            continue
+        cost = G._mask_cost([(c[1], c[2], c[3], c[0]) for c in chosen], r0)
+        if best is None or cost < best - 1e-12:
+            best = cost
+    assert abs(G._mask_cost(got, r0) - best) < 1e-9
+    assert len({fi for fi, _, _, _ in got}) == len(got) <= n
+    assert sum(-(-cost_of(m) // unit) for _, _, m, _ in got) <= room // unit
+
+
+def test_budgeted_masks_only_where_the_budget_binds(monkeypatch):
+    """k=397's default wheel (row-lane tables of 3m + 64 rows) hits the 64 KiB budget with the
+    greedy rule, which takes 41^2 = 1681 and is left with 0.02 candidate slots per outer index;
+    the knapsack fits six masks within the budget, cheaper by the cost model.  The k=399
+    primary wheel does not hit the budget and keeps the greedy's six masks."""
+    import gpu_sieve_mi355x as G
+    plan = EP.Plan(397)
+    d = G._Desc(plan, dict(plan.tun, mi_inner_radix=128))
+    assert d.rows
+    with monkeypatch.context() as mp:
+        mp.setattr(G, "_masks_within_budget", lambda *a, **kw: [(-1, 0, 0, 1.0)])  # greedy only
+        g = G._Desc(plan, dict(plan.tun, mi_inner_radix=128))
+    assert 1681 in [m for _, _, m, _ in g.masks]
+    assert sum(2 * (3 * m + 64) + 4 for _, _, m, _ in d.masks) <= 65536 // 4
+    assert len(d.masks) == 6 and G._mask_cost(d.masks, d.r0) < 9.0
+    assert G._mask_cost(d.masks, d.r0) < G._mask_cost(g.masks, g.r0) - 3.0
+    p399 = EP.Plan(399, None, [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1],
+                               [103, 1], [137, 1], [41, 1], [2, 9], [37, 1], [29, 1],
+                               [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]])
+    d399 = G._Desc(p399, dict(p399.tun, mi_inner_radix=128))
+    assert [m for _, _, m, _ in d399.masks] == [529, 223, 227, 229, 233, 149]
+
+
 def test_fullpower_wheel_k200():
     """Config-2 wheel: full powers for p <= 23 plus the best single-digit extension; no item
     left out fits both caps, and no other subset of the fitting items does better (brute force

@@ -26,6 +26,11 @@ K = 399
 WHEEL = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1], [41, 1],
          [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]]
 BLOCK = 6
+# the recorded word-mask confirmation wheel of k=399 (runs.json): r0 = 47, R = 1.755e16, g(399)
+# in block 4
+WHEEL_CONF = [[223, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+              [5, 4], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1]]
+BLOCK_CONF = 4
 G399 = 1463367633345557449307696345739199
 # INT32 VALU peak: 256 CUs x 128 lane-ops per CU-clock (two wave64 VALU instructions per clock,
 # the vector rate behind the 157.3 TFLOP/s FP32 figure) x 2.4 GHz.  `--workload valu` measures
@@ -47,6 +52,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-confirmation", action="store_true",
+                    help="sieve: leave out the k=399 confirmation wheel's rate")
     ap.add_argument("--no-part2", action="store_true",
                     help="sieve: leave out the surrogate-vs-plain time to first nonempty block")
     ap.add_argument("--autotune", action="store_true",
@@ -156,6 +163,7 @@ def main() -> int:
     lds_peak = lds_read_peak()
     roofline = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
     cpu = None if a.no_cpu else cpu_baseline(plan, eng, g_idx, a)
+    conf = None if a.no_confirmation else confirmation_wheel(a, lds_peak)
     part2 = None if a.no_part2 else first_block_ab()
 
     print(json.dumps({
@@ -168,9 +176,48 @@ def main() -> int:
                    "k": K, "r0": eng.r0, "R": plan.R, "window": N, "masks": F,
                    "strip": eng.strip_len, "threads": eng.threads, "layout": eng.layout,
                    "steps_per_unit": eng.info.steps if eng.layout == "rows" else None},
-        "roofline": roofline, "cpu_baseline": cpu, "time_to_first_block": part2,
+        "roofline": roofline, "cpu_baseline": cpu, "confirmation_wheel": conf,
+        "time_to_first_block": part2,
     }), flush=True)
     return 0
+
+
+def confirmation_wheel(a, lds_peak: dict, steps: int = 10) -> dict:
+    """The second k=399 wheel of the record (the word-mask confirmation run: ring 0 of 47
+    residues, which the engine may sweep behind its 24-residue ring as a lead block, whichever
+    order it times faster) on consecutive windows of its block 4, the last holding g(399):
+    positions per second of wall time and of kernel time, with its LDS roofline."""
+    import es_plan
+    import gpu_sieve_mi355x as G
+    plan = es_plan.Plan(K, None, WHEEL_CONF)
+    eng = G.GpuSieve(plan)
+    N = 1 << a.window_log2
+    g_idx, g_t = plan.n_to_idx(G399)
+    assert g_t == BLOCK_CONF
+    nsteps = 2 + steps
+    base = g_idx - (nsteps - 1) * N - N // 2
+    windows = [(base + i * N, base + (i + 1) * N) for i in range(nsteps)]
+    for w in windows[:2]:
+        eng.scan_window(BLOCK_CONF, *w)
+    eng.sync()
+    eng.kernel_time()
+    t0 = time.perf_counter()
+    found = [eng.scan_window(BLOCK_CONF, *w) for w in windows[2:]]
+    eng.sync()
+    wall = time.perf_counter() - t0
+    dev_ms, launches, dev_pos = eng.kernel_time()
+    assert found[-1][1] == G399, f"last window must return g(399), got {found[-1][1]}"
+    kern_rate = dev_pos / (dev_ms / 1e3)
+    roof = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
+    return {"value": steps * N / wall, "unit": "positions/s",
+            "config": {"workload": f"k=399 confirmation wheel (runs.json), block {BLOCK_CONF}, "
+                                   f"{steps} windows of 2^{a.window_log2} positions",
+                       "R": plan.R, "r0_plan": plan.radix[0], "r0_kernel": eng.info.r0,
+                       "lead_block": eng._d.lead, "ring_swap": eng.ring_swap,
+                       "layout": eng.layout, "masks": eng.info.n_masks_used},
+            "kernel_positions_per_s": kern_rate, "kernel_avg_launch_ms": dev_ms / max(1, launches),
+            "lds_bytes_per_position": roof["lds_bytes_per_position"],
+            "lds_frac": roof["frac"]}
 
 
 def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 313, 317),

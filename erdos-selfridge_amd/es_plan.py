@@ -55,8 +55,10 @@ TUNABLES = {
                                # rows grouped by LDS bank ("auto": rows where they apply)
     "mi_inner_radix": "auto",  # ring 0 mod p^T wider than this is swept as digit rings (0:
                                # never; "auto": the engine times the distinct splits)
-    "mi_ring_swap": "auto",    # "auto": a ring 0 of 33..128 residues trades places with a
-                               # ring 1 of <= 32 in the kernel (one-word mask rows); 0: never
+    "mi_ring_swap": "auto",    # a ring 0 of 33..128 residues trading places with the first
+                               # later ring of <= 32 in the kernel (one-word mask rows): 1
+                               # wherever possible, 0 never, "rule" where it fills its word
+                               # clearly better, "auto": the engine times both orders
 }
 
 

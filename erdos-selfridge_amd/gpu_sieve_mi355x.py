@@ -90,15 +90,17 @@ LEAD_CAP = 8192   # positions of the lead block (the rings before the one brough
 LEAD_GAIN = 1.15  # least gain in mask-word use (slots per word bit) that brings a ring forward
 
 
-def swap_lead(radix, inner_mod: int = 0) -> int:
+def swap_lead(radix, inner_mod: int = 0, force: bool = False) -> int:
     """j >= 1 when the kernel sweeps the plan's ring j as its ring 0 (one word per mask row)
     and the rings 0..j-1 before it as its first strip rings (the lead block): when ring 0 has
     33..128 residues (two to four words per row) and is not split into digit rings, ring j is
-    the first ring of at most 32, the lead block holds at most LEAD_CAP positions, and ring j
-    fills its word at least LEAD_GAIN times as well as ring 0 fills its words (or ring 0 needs
-    four words).  0 otherwise.  idx keeps the plan's order; the engine maps it.  (Measured on
-    2^36 windows: k=302/308/309/312/386 +35..45 %, k=378/391 1.7x/3.5x; k=360/380/392/400, whose
-    ring 0 fills its words as well as ring j would, lose 6..12 % swapped, so they keep it.)"""
+    the first ring of at most 32, the lead block holds at most LEAD_CAP positions, and (unless
+    `force`) ring j fills its word at least LEAD_GAIN times as well as ring 0 fills its words
+    (or ring 0 needs four words).  0 otherwise.  idx keeps the plan's order; the engine maps
+    it.  The fill rule is a cheap guess: GpuSieve times both orders where a swap is possible
+    (on 2^36 windows of the 84 frontier wheels that can swap, the rule took the slower order
+    by more than 5 % for 11: k=372/383 run 1.31-1.33x faster swapped against it, k=325 1.12x
+    faster unswapped; k=378c/400c would lose 20 % swapped and the rule keeps them)."""
     if inner_mod or len(radix) < 2 or not 32 < radix[0] <= 128:
         return 0
     L = 1
@@ -107,9 +109,22 @@ def swap_lead(radix, inner_mod: int = 0) -> int:
         if L > LEAD_CAP:
             return 0
         if radix[j] <= 32:
+            if force:
+                return j
             fill0 = radix[0] / (32 * -(-radix[0] // 32))
             return j if radix[0] > 96 or radix[j] / 32 >= LEAD_GAIN * fill0 else 0
     return 0
+
+
+def swap_mode(tun: dict):
+    """The mi_ring_swap tunable as 0, 1, "rule" or "auto" (ints may come as strings)."""
+    v = tun.get("mi_ring_swap", "auto")
+    if v in ("auto", "rule"):
+        return v
+    v = int(v)
+    if v not in (0, 1):
+        raise ValueError(f"mi_ring_swap must be 'auto', 'rule', 0 or 1, not {v!r}")
+    return v
 
 
 def lead_order(n: int, j: int) -> list[int]:
@@ -311,8 +326,11 @@ class _Desc:
         mods, radix, rterms, self.inner_mod, self.f = kernel_rings(
             plan, int(tun.get("mi_inner_radix", 128)))
         self.lead = 0
-        j = (swap_lead(radix, self.inner_mod) if tun.get("mi_ring_swap", "auto") == "auto"
-             and not int(tun.get("mi_global_masks", 0)) else 0)
+        # "auto" reaches here only from callers that build a descriptor without timing (a
+        # GpuSieve resolves it to 0 or 1 first): it then follows the fill rule
+        mode = swap_mode(tun)
+        j = (swap_lead(radix, self.inner_mod, force=mode == 1)
+             if mode != 0 and not int(tun.get("mi_global_masks", 0)) else 0)
         if j:
             self.lead = math.prod(radix[:j])
             order = lead_order(len(radix), j)
@@ -496,9 +514,9 @@ class GpuSieve:
         P = plan
         self._h = None
         L = nat.lib()
-        inner = P.tun.get("mi_inner_radix", "auto")
-        self.inner_bound = self._choose_inner(P) if inner == "auto" else int(inner)
-        self._d = _Desc(P, dict(P.tun, mi_inner_radix=self.inner_bound))
+        self.inner_bound, self.ring_swap = self._choose_layout(P)
+        self._d = _Desc(P, dict(P.tun, mi_inner_radix=self.inner_bound,
+                                mi_ring_swap=self.ring_swap))
         h = C.c_void_p()
         nat.check(L.es_engine_create(C.byref(self._d.desc), C.byref(h)))
         self._h = h
@@ -541,26 +559,50 @@ class GpuSieve:
             self._h = None
 
     @staticmethod
-    def _choose_inner(P: Plan, strips_log2: int = 19) -> int:
-        """Digit-split bound for a ring 0 wider than 64 slots mod p^T (T >= 2): time every
-        distinct kernel inner radix the split can give (whole ring, groups <= 128, 64, 32, 16)
-        on a window of block 1 and keep the fastest.  Every choice scans the same positions
-        with the same exact test; only the LDS layout (words per row, masks that fit) and so
-        the speed differ.  Each window holds 2^strips_log2 strips of its own layout (about 8
-        workgroups per CU, so long strips are not timed on an under-filled GPU), between 2^30
-        and 2^34 positions."""
-        if not P.rings or P.radix[0] <= 64 or P.wheel[0][1] < 2:
-            return 128
-        opts = {}
-        for bound in (128, 64, 32, 16, 0):
-            opts.setdefault(kernel_rings(P, bound)[1][0], bound)
+    def _layout_options(P: Plan) -> list[tuple[int, object]]:
+        """Distinct kernel ring layouts of a plan as (mi_inner_radix, mi_ring_swap) pairs:
+        the digit-split bounds that give distinct kernel inner radices (a ring 0 wider than 64
+        slots mod p^T, T >= 2: whole ring, groups <= 128, 64, 32, 16), and for each layout
+        whose ring 0 stays whole and could trade places with a later ring of <= 32 residues,
+        both orders.  Tunables other than "auto" fix their dimension."""
+        inner = P.tun.get("mi_inner_radix", "auto")
+        swap = swap_mode(P.tun)
+        if inner != "auto":
+            bounds = [int(inner)]
+        elif not P.rings or P.radix[0] <= 64 or P.wheel[0][1] < 2:
+            bounds = [128]
+        else:
+            bounds = [128, 64, 32, 16, 0]
+        opts, seen = [], set()
+        for bound in bounds:
+            mods, radix, _, inner_mod, _ = kernel_rings(P, bound)
+            if radix[0] in seen:
+                continue
+            seen.add(radix[0])
+            can = (not int(P.tun.get("mi_global_masks", 0)) and
+                   swap_lead(radix, inner_mod, force=True) > 0)
+            if swap != "auto" or not can:
+                opts.append((bound, swap if swap != "auto" else 0))
+            else:
+                opts += [(bound, 0), (bound, 1)]
+        return opts
+
+    @staticmethod
+    def _choose_layout(P: Plan, strips_log2: int = 19) -> tuple[int, object]:
+        """(mi_inner_radix, mi_ring_swap) for the engine: with one layout option, that one;
+        otherwise each option is timed on a window of block 1 and the fastest kept.  Every
+        option scans the same positions with the same exact test; only the LDS layout (words
+        per row, masks that fit, strip-lane or row-lane lanes) and so the speed differ.  Each
+        window holds 2^strips_log2 strips of its own layout (about 8 workgroups per CU, so long
+        strips are not timed on an under-filled GPU), between 2^28 and 2^34 positions."""
+        opts = GpuSieve._layout_options(P)
         if len(opts) == 1:
-            return 128
+            return opts[0]
         L = nat.lib()
-        best, best_rate = 128, 0.0
-        for bound in opts.values():
+        best, best_rate = opts[0], 0.0
+        for bound, swap in opts:
             try:
-                d = _Desc(P, dict(P.tun, mi_inner_radix=bound))
+                d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap))
             except AssertionError:
                 continue
             h = C.c_void_p()
@@ -582,7 +624,7 @@ class GpuSieve:
                 rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05,
                                   floor=0.5 * best_rate)
                 if rate > best_rate:
-                    best, best_rate = bound, rate
+                    best, best_rate = (bound, swap), rate
             finally:
                 L.es_engine_destroy(h)
         return best

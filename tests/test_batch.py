@@ -19,12 +19,14 @@ def _engine(k, **tune):
 
 def test_batch_equals_unbatched_dense():
     """Filter-prefix engines (dense survivors) of mixed shapes: one-word rows, swapped lead
-    rings (k=301, 308) and two-word rows (k=301 kept in the plan's ring order), several F."""
+    rings (k=301, 308, forced) and two-word rows (k=301 kept in the plan's ring order), several
+    F."""
     import es_batch
     rng = random.Random(2026)
     b = es_batch.Batch(out_cap_per_job=1 << 20)
     jobs = []
-    for k, tune in ((300, {}), (301, {}), (301, {"mi_ring_swap": 0}), (308, {}), (317, {})):
+    for k, tune in ((300, {}), (301, {"mi_ring_swap": 1}), (301, {"mi_ring_swap": 0}),
+                    (308, {"mi_ring_swap": 1}), (317, {})):
         eng = _engine(k, mi_filter_limit=4, max_survivors=1 << 20, **tune)
         for _ in range(2):
             t = rng.randrange(1 << 20)

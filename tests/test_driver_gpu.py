@@ -58,11 +58,12 @@ def test_driver_max_blocks_then_resume(tmp_path):
 
 
 def test_driver_max_launches_mid_block_then_resume(tmp_path):
-    """k=309 (its wheel's ring 0 of 42 gives way to a lead block): pause after a few launches
-    inside block 0, resume twice, and require the uninterrupted run's audit rows and hashes."""
+    """k=309 (its wheel's ring 0 of 42 gives way to a lead block, forced): pause after a few
+    launches inside block 0, resume twice, and require the uninterrupted run's audit rows and
+    hashes."""
     k = 309
     say(f"case driver k={k} max-launches mid-block")
-    tun = {"mi_launch_log2": 40}
+    tun = {"mi_launch_log2": 40, "mi_ring_swap": 1}
     full = drive(tmp_path / "full", k, tun=tun, resume=False)
     assert full["status"] == "FOUND" and int(full["g"]) == bfile()[k]
     leg = drive(tmp_path / "leg", k, tun=tun, max_launches=3, resume=False)

@@ -147,20 +147,36 @@ def _frontier():
     return json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
 
 
-_RUNS = [(t["k"], role) for t in json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
-         for role in ("primary", "confirmation")]
+def _recorded_runs():
+    """(k, role, mi_ring_swap): both forced orders for a wheel whose ring 0 could trade places
+    with a later ring (the engine's timing picks one of them), the default otherwise."""
+    import gpu_sieve_mi355x as G
+    out = []
+    for t in _frontier():
+        for role in ("primary", "confirmation"):
+            P = EP.Plan(t["k"], None, t[role]["wheel"])
+            opts = G.GpuSieve._layout_options(P)
+            swaps = sorted({s for _, s in opts}) if len(opts) > 1 else ["auto"]
+            out += [(t["k"], role, s) for s in swaps]
+    return out
 
 
-@pytest.mark.parametrize("k,role", _RUNS)
-def test_recorded_survivor_windows(k, role):
-    """Every recorded run of evidence/through400/runs.json (46 wheels: both roles of k=378..400):
-    around every recorded survivor of every block, a 2^24-position window returns exactly the
-    recorded survivors inside it and the oracle's survivors, and its minimum agrees with the
-    oracle's (full filter set)."""
+@pytest.mark.parametrize("k,role,swap", _recorded_runs())
+def test_recorded_survivor_windows(k, role, swap):
+    """Every recorded run of evidence/through400/runs.json (46 wheels: both roles of k=378..400),
+    in both kernel ring orders where its ring 0 could give way to a lead block: around every
+    recorded survivor of every block, a 2^24-position window returns exactly the recorded
+    survivors inside it and the oracle's survivors, and its minimum agrees with the oracle's
+    (full filter set)."""
     term = next(x for x in _frontier() if x["k"] == k)
     run = term[role]
-    eng, plan = engine(k, run["wheel"], max_survivors=1 << 16)
+    # a forced order also keeps ring 0 whole (k=378's 3^6 ring could otherwise be timed as
+    # digit rings, which have no lead block)
+    eng, plan = engine(k, run["wheel"], max_survivors=1 << 16,
+                       **({} if swap == "auto" else {"mi_ring_swap": swap, "mi_inner_radix": 128}))
     assert str(plan.M) == run["N"] and plan.R == run["residues_per_block"]
+    if swap != "auto":
+        assert (eng._d.lead > 0) == (swap == 1)
     windows = 0
     for audit in run["block_audit"]:
         t = audit["t"]
@@ -320,20 +336,23 @@ def test_carry_table_forms(k, tune):
     window_check(k, wheel, 3, lo, lo + span, limit=8, **tune)
 
 
-@pytest.mark.parametrize("k", [301, 309, 312, 378, 363, 386])
+@pytest.mark.parametrize("k", [301, 309, 312, 378, 363, 386, 372, 383])
 def test_swapped_lead_rings(k):
     """A wheel whose ring 0 has 33..128 residues runs with its first ring j of at most 32
     residues as the kernel's ring 0 and rings 0..j-1 as a lead block behind it (j = 1, or 2 at
-    k=363 and 386), one-word mask rows, while idx keeps the plan's order: unaligned windows
-    (with their minimum), outer-unit launches and the unswapped engine all give the oracle's
-    survivors.  (k=378's ring 0 is 3^6 with 108 residues: the automatic inner radix also times
-    its digit splits, so the bound is fixed at 128 here to keep it whole.)"""
+    k=363, 383 and 386), one-word mask rows, while idx keeps the plan's order: unaligned
+    windows (with their minimum), outer-unit launches and the unswapped engine all give the
+    oracle's survivors.  Both orders are forced (mi_ring_swap 1 and 0), whatever the engine's
+    timing would choose; k=372 and 383 are wheels the fill rule alone would not swap.  (k=378's
+    ring 0 is 3^6 with 108 residues: the inner radix is fixed at 128 to keep it whole.)"""
     import gpu_sieve_mi355x as G
     keep = {"mi_inner_radix": 128} if k == 378 else {}
+    keep["mi_ring_swap"] = 1
     eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, **keep)
     say(f"case k={k} radix {plan.radix[:3]} kernel r0={eng._d.r0} lead={eng._d.lead}")
-    j = G.swap_lead(plan.radix)
+    j = G.swap_lead(plan.radix, force=True)
     assert j and eng._d.lead == np.prod(plan.radix[:j]) and eng._d.r0 == plan.radix[j] <= 32
+    assert (eng.inner_bound, eng.ring_swap) == (keep.get("mi_inner_radix", 128), 1)
     rng = random.Random(k)
     for span in (1 << 22, 12345, 1):
         lo = rng.randrange(0, plan.R - span)
@@ -350,25 +369,51 @@ def test_swapped_lead_rings(k):
     assert n == nw and np.array_equal(a, want)
     b, _ = eng.scan_window(4, lo, hi)
     assert np.array_equal(np.sort(np.asarray(b, dtype=np.uint64)), want)
-    eng2, _ = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, mi_ring_swap=0, **keep)
+    keep["mi_ring_swap"] = 0
+    eng2, _ = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, **keep)
     assert eng2._d.lead == 0
     c, _ = eng2.scan_window(4, lo, hi)
     assert np.array_equal(np.sort(np.asarray(c, dtype=np.uint64)), want)
+
+
+@pytest.mark.parametrize("k,wheel", [(399, K399_CONFIRM), (383, None), (400, None)])
+def test_timed_ring_order(k, wheel):
+    """With mi_ring_swap "auto" the engine times both kernel ring orders of a wheel whose ring 0
+    could give way to a lead block (k=399's confirmation wheel, 47 then 24; k=383 and 400's
+    default wheels, lead blocks of two rings) and keeps the faster: its descriptor follows the
+    choice, and the chosen engine scans the oracle's survivors."""
+    import gpu_sieve_mi355x as G
+    tune = {"mi_filter_limit": 8, "max_survivors": 1 << 22}
+    plan = EP.Plan(k, dict(tune), wheel)
+    assert len(G.GpuSieve._layout_options(plan)) >= 2
+    eng = G.GpuSieve(plan)
+    say(f"case timed ring order k={k}: swap={eng.ring_swap} lead={eng._d.lead} "
+        f"layout={eng.layout} r0={eng.info.r0}")
+    assert eng.ring_swap in (0, 1) and (eng._d.lead > 0) == (eng.ring_swap == 1)
+    rng = random.Random(31 * k)
+    span = min(1 << 22, plan.R // 2)
+    lo = rng.randrange(0, plan.R - span)
+    ids, mn = eng.scan_window(5, lo, lo + span)
+    n, want, want_min = O.c_scan_min(k, plan.wheel, plan.rest[:8], 5, lo, lo + span, threads=16,
+                                     cap=1 << 22)
+    assert sorted(ids) == [int(x) for x in want] and mn == want_min
 
 
 @pytest.mark.parametrize("k", [100, 309])
 def test_window_edges(k):
     """Edge windows against the oracle: empty, the block's first and last positions, and a
     window crossing an outer-index boundary by one position (k=309 runs swapped lead rings,
-    whose windows cover whole runs of lead * r0 positions)."""
-    eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22)
+    forced, whose windows cover whole runs of lead * r0 positions)."""
+    tune = {"mi_ring_swap": 1} if k == 309 else {}
+    eng, plan = engine(k, None, mi_filter_limit=8, max_survivors=1 << 22, **tune)
+    assert (eng._d.lead > 0) == (k == 309)
     ids, mn = eng.scan_window(2, 1000, 1000)
     assert list(ids) == [] and mn is None
     r = plan.radix[0]
     span = min(1 << 20, plan.R // 2)
     for lo, hi in ((0, span), (plan.R - span, plan.R), (plan.R - 3, plan.R),
                    (5 * r - 1, 5 * r + 1)):
-        window_check(k, None, 2, lo, hi, limit=8)
+        window_check(k, None, 2, lo, hi, limit=8, **tune)
 
 
 def test_block_overflow_falls_back_to_per_launch_drain():

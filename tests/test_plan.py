@@ -284,6 +284,46 @@ def test_lead_block_rule():
     assert G.swap_lead([108, 26, 20]) == 1           # four words
     assert G.swap_lead([40, 30], inner_mod=3 ** 6) == 0 and G.swap_lead([30, 20]) == 0
     assert G.swap_lead([100, 90, 90, 20]) == 0       # lead block above the cap
+    # forced: wherever a ring of <= 32 can lead within the cap, whatever the fill
+    assert G.swap_lead([64, 45, 32, 26], force=True) == 2
+    assert G.swap_lead([77, 36, 24, 18], force=True) == 2
+    assert G.swap_lead([47, 24, 23], force=True) == 1 and G.swap_lead([47, 24, 23]) == 0
+    assert G.swap_lead([100, 90, 90, 20], force=True) == 0
+    assert G.swap_lead([40, 30], inner_mod=3 ** 6, force=True) == 0
+    assert G.swap_lead([30, 20], force=True) == 0
+
+
+K399_CONFIRM = [[223, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+                [5, 4], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1]]
+K399_PRIMARY = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+                [41, 1], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]]
+
+
+def test_layout_options_the_engine_times():
+    """GpuSieve times the distinct kernel ring layouts of a plan: both orders where ring 0
+    (33..128 residues, whole) could trade places with a later ring of <= 32 (the k=399
+    confirmation wheel: 47 then 24), one layout where it cannot (the primary wheel, r0 = 29),
+    digit splits of a wide ring 0 mod p^T only unswapped (k=378: 3^6, 108 residues), and a fixed
+    tunable fixes its dimension.  The descriptor follows the chosen order."""
+    import gpu_sieve_mi355x as G
+    opts = lambda k, w=None, **t: G.GpuSieve._layout_options(EP.Plan(k, t, w))
+    assert opts(399, K399_PRIMARY) == [(128, 0)]
+    assert opts(399, K399_CONFIRM) == [(128, 0), (128, 1)]
+    assert opts(399, K399_CONFIRM, mi_ring_swap=0) == [(128, 0)]
+    assert opts(399, K399_CONFIRM, mi_ring_swap="1") == [(128, 1)]
+    assert opts(399, K399_CONFIRM, mi_ring_swap="rule") == [(128, "rule")]
+    assert opts(399, K399_CONFIRM, mi_global_masks=2) == [(128, 0)]
+    o378 = opts(378)
+    assert o378[:2] == [(128, 0), (128, 1)] and len(o378) > 2
+    assert all(s == 0 for b, s in o378[2:])
+    assert opts(378, mi_inner_radix=64) == [(64, 0)]
+    with pytest.raises(ValueError):
+        opts(399, K399_CONFIRM, mi_ring_swap=2)
+    P = EP.Plan(399, None, K399_CONFIRM)
+    d1 = G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap=1))
+    d0 = G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap=0))
+    dr = G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap="rule"))
+    assert (d1.lead, d1.r0) == (47, 24) and (d0.lead, d0.r0) == (0, 47) and dr.lead == 0
 
 
 def test_target_g_plans_reproduced():

@@ -52,6 +52,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--part2-candidates", type=int, default=10000,
+                    help="sieve: candidate wheels ranked per k for the time to first block")
+    ap.add_argument("--part2-shortlist", type=int, default=8,
+                    help="sieve: wheels probed per k (besides the plain one) for it")
     ap.add_argument("--no-confirmation", action="store_true",
                     help="sieve: leave out the k=399 confirmation wheel's rate")
     ap.add_argument("--no-part2", action="store_true",
@@ -164,7 +168,8 @@ def main() -> int:
     roofline = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
     cpu = None if a.no_cpu else cpu_baseline(plan, eng, g_idx, a)
     conf = None if a.no_confirmation else confirmation_wheel(a, lds_peak)
-    part2 = None if a.no_part2 else first_block_ab()
+    part2 = None if a.no_part2 else first_block_ab(candidates=a.part2_candidates,
+                                                   shortlist=a.part2_shortlist)
 
     print(json.dumps({
         "metric": "residue positions sieved/s (k=399 wheel); time to first nonempty block vs plain",
@@ -221,14 +226,14 @@ def confirmation_wheel(a, lds_peak: dict, steps: int = 10) -> dict:
 
 
 def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 313, 317),
-                   pairs: int = 2) -> dict:
+                   pairs: int = 2, candidates: int = 10000, shortlist: int = 8) -> dict:
     """Metric part 2 measured end to end: time to exhaust the first nonempty block with the
     surrogate-planned wheel against the plain planner's wheel, on k whose first nonempty block
     scans in about a second (the frontier k = 350..400 take hours per block: bench.py
     --workload surrogate estimates those from measured rates).  The network is trained on
-    wheels the sieve probes at even k = 300..320; per odd k (not trained on) it ranks 10^4
-    candidate wheels, the first 8 and the plain wheel are probed, and the least E_pos x ns
-    wins.  Both wheels then run the full search (GpuSieve.search: blocks t = 0, 1, ... until
+    wheels the sieve probes at even k = 300..320; per odd k (not trained on) it ranks
+    `candidates` wheels, the first `shortlist` and the plain wheel are probed, and the least
+    E_pos x ns wins.  Both wheels then run the full search (GpuSieve.search: blocks t = 0, 1, ... until
     the referee accepts a survivor) in interleaved pairs (plain, surrogate, surrogate, plain,
     ...; harness/paired_bench.py:22-68), each must return the b-file g(k), and each wheel's
     time is its median over the pairs.  Engine construction is outside the timed search."""
@@ -245,7 +250,8 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
     t_train = time.perf_counter() - t0
     rows = []
     for k in eval_ks:
-        res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=8, probe_log2=33, seed=k)
+        res = S.surrogate_plan(k, model, n_candidates=candidates, shortlist=shortlist,
+                               probe_log2=33, seed=k)
         wheels = {"plain": res["plain"]["wheel"], "surrogate": res["best"]["wheel"]}
         engines = {tag: G.GpuSieve(es_plan.Plan(k, None, w)) for tag, w in wheels.items()}
         times = {"plain": [], "surrogate": []}
@@ -274,7 +280,7 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
             "config": {"workload": f"searches of k={','.join(map(str, eval_ks))} from block 0 "
                                    f"to g(k); network trained on k={train_ks[0]}..{train_ks[-1]}"
                                    f" even, {len(X)} probed wheels", "pairs": pairs,
-                       "candidates": 10000, "shortlist": 8},
+                       "candidates": candidates, "shortlist": shortlist},
             "train_and_plan_s": t_train, "per_k": rows}
 
 

@@ -72,19 +72,19 @@ KernelFn pick_kernel(int F, uint32_t W, uint32_t G, uint32_t C) {
   if (F < kPartFirstF[3]) return pick_kernel_p2(F, W, G, C);
   return pick_kernel_p3(F, W, G, C);
 }
-RowsFn pick_rows(int F, uint32_t C) {
+RowsFn pick_rows(int F, uint32_t C, uint32_t W) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (F < kPartFirstF[1]) return pick_rows_p0(F, C);
-  if (F < kPartFirstF[2]) return pick_rows_p1(F, C);
-  if (F < kPartFirstF[3]) return pick_rows_p2(F, C);
-  return pick_rows_p3(F, C);
+  if (F < kPartFirstF[1]) return pick_rows_p0(F, C, W);
+  if (F < kPartFirstF[2]) return pick_rows_p1(F, C, W);
+  if (F < kPartFirstF[3]) return pick_rows_p2(F, C, W);
+  return pick_rows_p3(F, C, W);
 }
-RowsBatchFn pick_rows_batch(int F, uint32_t C) {
+RowsBatchFn pick_rows_batch(int F, uint32_t C, uint32_t W) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (F < kPartFirstF[1]) return pick_rows_batch_p0(F, C);
-  if (F < kPartFirstF[2]) return pick_rows_batch_p1(F, C);
-  if (F < kPartFirstF[3]) return pick_rows_batch_p2(F, C);
-  return pick_rows_batch_p3(F, C);
+  if (F < kPartFirstF[1]) return pick_rows_batch_p0(F, C, W);
+  if (F < kPartFirstF[2]) return pick_rows_batch_p1(F, C, W);
+  if (F < kPartFirstF[3]) return pick_rows_batch_p2(F, C, W);
+  return pick_rows_batch_p3(F, C, W);
 }
 
 // Lane slots of the strip rows for the row-lane layout: G groups of 64 slots, i.e. 2G
@@ -469,16 +469,18 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // Row-lane layout: the rings after the strip rings, while their radix product stays at
   // most 1024, are the sub rings a wave steps through (V steps per unit: short enough that a
   // window holds many units, so the last round of waves is short); the rest are the
-  // superstrip rings.  It applies to one-word rows with every mask in LDS, a strip of at
-  // least 32 rows and at least 16 steps.
+  // superstrip rings.  It applies with every mask in LDS, a strip of at least 32 rows and at
+  // least 16 steps, to one-word rows by default and to two-word rows (ring 0 of 33..64 slots)
+  // on request (layout 2: which is faster depends on the wheel, so the caller times both).
   if (d->layout > 2) return fail(ES_ERR_INVALID, "layout must be 0, 1 or 2");
   uint32_t n_sub = 0;
   uint64_t V = 1;
   while (1 + m + n_sub < w && V * d->ring_radix[1 + m + n_sub] <= 1024) V *= d->ring_radix[1 + m + n_sub++];
-  bool rows = d->layout != 1 && W == 1 && d->global_masks == 0 && S >= 32 && V >= 16;
+  bool rows = (d->layout == 0 ? W == 1 : d->layout == 2 && W <= (uint32_t)kRowsMaxWords) &&
+              d->global_masks == 0 && S >= 32 && V >= 16;
   if (d->layout == 2 && !rows)
-    return fail(ES_ERR_LIMIT, "row-lane layout needs one-word rows, every mask in LDS, a strip "
-                              "of at least 32 rows and at least 16 steps");
+    return fail(ES_ERR_LIMIT, "row-lane layout needs rows of at most two words, every mask in "
+                              "LDS, a strip of at least 32 rows and at least 16 steps");
   e->rows = rows;
   e->V = rows ? (uint32_t)V : 1;
   e->Gl = rows ? (uint32_t)((S + 63) / 64) : 0;
@@ -498,20 +500,25 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // bank, so its random reads never conflict (7 % faster at k=399, q0 = 107, than word rows;
   // with q0 > 256 the bytes conflict too and the extra instructions cost 3 %).  Otherwise W <=
   // 2: one row of W words per h (bit i set iff j(h) <= i < r0).  W > 2: j(h) as u16, each
-  // carry word built from it (word rows would not fit).
-  const uint32_t jbytes = (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
+  // carry word built from it (word rows would not fit).  Row-lane rows of two words: j(h) <=
+  // 64 as one byte per h, each word's carry built from it.
+  auto carry_form = [&](bool rw) -> uint32_t {
+    if (rw && W >= 2) return 1u;
+    return (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
+  };
+  uint32_t jbytes = carry_form(rows);
   e->kind = (jbytes == 1 ? 1u : 0u) | (e->lead > 1 ? 2u : 0u);
   if (e->lead > 1 && (W != 1 || d->global_masks != 0))
     return fail(ES_ERR_INVALID, "a lead block needs a ring 0 of at most 32 residues and "
                                 "every mask in LDS");
   if (r0 > 0xffffu) return fail(ES_ERR_LIMIT, "ring-0 radix above 65535");
-  const uint32_t carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
+  uint32_t carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
   std::vector<uint32_t> staged;  // indices into the descriptor's mask list
   {
     size_t used = carry_words + 8;
     const size_t cap_masks = rows ? (size_t)kRowsMaxMasks : (size_t)kMaxMasks;
     for (uint32_t f = 0; f < d->n_masks && staged.size() < cap_masks; ++f) {
-      const size_t rows_words = rows ? (size_t)2 * (3 * (size_t)d->mask_mod[f] + 64) + 4
+      const size_t rows_words = rows ? (size_t)2 * W * (3 * (size_t)d->mask_mod[f] + 64) + 4
                                      : (size_t)4 * d->mask_mod[f] * W + 4;
       if (used + rows_words > budget_words) continue;  // stays exact-only
       if (inv_mod((uint32_t)((d->mask_mod[f] - mod_u128(M, d->mask_mod[f])) % d->mask_mod[f]),
@@ -527,6 +534,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     e->rows = false;
     e->V = 1;
     e->Gl = 0;
+    jbytes = carry_form(false);
+    e->kind = (jbytes == 1 ? 1u : 0u) | (e->lead > 1 ? 2u : 0u);
+    carry_words = jbytes ? (q0 * jbytes + 3) / 4 : q0 * W;
   }
   const uint32_t nb = rows ? w - 1 - m - n_sub : w - 1 - m;
   const uint32_t first_base = rows ? 1 + m + n_sub : 1 + m;
@@ -558,7 +568,20 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     }
   }
   std::vector<uint32_t> lds, gimg, mbase, mmod, msrc, minv;
-  if (rows) lds.assign(carry_words, 0);  // row-lane: the carry table opens the image
+  // Row-lane: the carry table opens the image, then block 0 of the tables (word 0 of each
+  // mask, 16-byte aligned), then blocks 1..W-1 at the same offsets, blk_words apart.
+  size_t blk_base = 0, blk_words = 0;
+  std::vector<size_t> blk_off(order_masks.size(), 0);
+  if (rows) {
+    size_t o = ((size_t)carry_words + 3) & ~size_t(3);
+    blk_base = o;
+    for (size_t pos = 0; pos < order_masks.size(); ++pos) {
+      blk_off[pos] = o - blk_base;
+      o = (o + (size_t)2 * (3 * (size_t)d->mask_mod[order_masks[pos]] + 64) + 3) & ~size_t(3);
+    }
+    blk_words = o - blk_base;
+    lds.assign(blk_base + (size_t)W * blk_words, 0);
+  }
   for (size_t pos = 0; pos < order_masks.size(); ++pos) {
     const uint32_t f = order_masks[pos];
     const bool in_global = pos + G >= order_masks.size();
@@ -572,9 +595,13 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     const uint32_t lam = rows && mq % 2 ? (uint32_t)((uint64_t)32 * cinv % mq) : cinv;
     const uint32_t xstep = inv_mod(lam, mq);            // phase of row 1
     const size_t n_rows = rows ? (size_t)3 * mq + 64 : (size_t)2 * mq;
-    const size_t rows_words = n_rows * 2 * W;
-    const size_t off = (img.size() + 3) & ~size_t(3);  // 16-byte aligned rows
-    img.resize(off + rows_words, 0);
+    size_t off;
+    if (rows) {
+      off = blk_base + blk_off[pos];
+    } else {
+      off = (img.size() + 3) & ~size_t(3);  // 16-byte aligned rows
+      img.resize(off + n_rows * 2 * W, 0);
+    }
     const uint32_t* bm = d->mask_bits + d->mask_off[f];
     std::vector<uint32_t> t0m(r0);
     for (uint32_t i = 0; i < r0; ++i) t0m[i] = mod_u128(e->terms[order[i]], mq);
@@ -585,7 +612,8 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
         const uint32_t b = (uint32_t)(((uint64_t)a + corr) % mq);
         const uint32_t wi = i >> 5, bit = 1u << (i & 31);
         for (size_t r = P; r < n_rows; r += mq) {
-          uint32_t* row = &img[off + (size_t)2 * W * r + 2 * wi];
+          uint32_t* row = rows ? &img[off + wi * blk_words + 2 * r]
+                               : &img[off + (size_t)2 * W * r + 2 * wi];
           if ((bm[a >> 5] >> (a & 31)) & 1u) row[0] |= bit;
           if ((bm[b >> 5] >> (b & 31)) & 1u) row[1] |= bit;
         }
@@ -688,8 +716,9 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     row[3] = (uint32_t)(L >> 32);
     row[4] = (uint32_t)(L >> 64);
     row[5] = (uint32_t)(L >> 96);
-    for (uint32_t f = 0; f < F; ++f)  // row offset of phase B in table f
-      row[8 + f] = 8u * W * (uint32_t)((uint64_t)mod_u128(B, mmod[f]) * minv[f] % mmod[f]);
+    for (uint32_t f = 0; f < F; ++f)  // row offset of phase B in table f (row-lane: word 0's)
+      row[8 + f] = (rows ? 8u : 8u * W) *
+                   (uint32_t)((uint64_t)mod_u128(B, mmod[f]) * minv[f] % mmod[f]);
     for (uint32_t j = 0; j < m; ++j) {  // odometer
       if (++dig[j] < d->ring_radix[1 + j]) break;
       dig[j] = 0;
@@ -785,7 +814,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
   if (rows) {
-    e->rows_kern = pick_rows((int)e->F, e->kind);
+    e->rows_kern = pick_rows((int)e->F, e->kind, W);
     if (!e->rows_kern) return fail(ES_ERR_LIMIT, "no row-lane kernel for this mask count");
     e->kern = nullptr;
   } else {
@@ -884,6 +913,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     Q.carry_base = P.carry_base;
     Q.stride = stride;
     Q.last_even = last_even;
+    Q.word_bytes = (uint32_t)(blk_words * 4);
     Q.lds_src = P.lds_src;
     Q.perm = (const uint32_t*)(base + o_perm);
     Q.strip = P.strip;
@@ -1293,7 +1323,7 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   std::vector<Key> keys;
   for (uint32_t i = 0; i < b->jobs.size(); ++i) {
     const es_engine* e = b->jobs[i].e;
-    const void* fn = e->rows ? (const void*)pick_rows_batch((int)e->F, e->kind)
+    const void* fn = e->rows ? (const void*)pick_rows_batch((int)e->F, e->kind, e->W)
                              : (const void*)pick_grid_batch((int)e->F, e->W, 0, e->kind);
     if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
     const uint32_t occ = std::min<uint32_t>(2048u / e->threads,

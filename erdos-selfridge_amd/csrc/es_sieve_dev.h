@@ -504,8 +504,18 @@ es_grid_batch_kernel(const HotPlan* __restrict__ plans, const GridJob* __restric
 // stored; only Q_sg and c_u differ between the lanes of a wave, and c_u keeps the bank.
 // (Reducing the lane's row mod m at unit set-up would halve that but moves the lanes whose
 // sum passes m by -m rows, to another bank: 1.5x the conflict cycles at k=399, measured.)
+// A ring 0 of two words (33..64 slots) keeps one such table per word (8-byte rows: the word
+// of x and of x - M); the word-1 tables of all masks form a second block, word_bytes after
+// the first, so a lane's word-1 row sits at the same bank offset from its word-0 row in every
+// table and the grouping serves both words.  (Three and four words were built too: their
+// instances need 96 VGPRs and spill, and they ran 0.64-0.98x the strip-lane layout on every
+// frontier wheel measured, so they are not instantiated.)
 constexpr int kRowsMaxMasks = 11;   // a step row: h, z (4 dwords), then one row offset per mask
 constexpr int kStepDwords = 16;
+constexpr int kRowsMaxWords = 2;
+// Row-lane queue entry: {bits, v | w << kRowsWordShift | lane << kQueueIndexBits} (v < 1024:
+// steps per unit; w: the ring-0 word)
+constexpr uint32_t kRowsWordShift = 10;
 
 struct RowPlan {
   uint64_t M0_lo, M0_hi;
@@ -515,6 +525,7 @@ struct RowPlan {
   uint32_t stride;                 // dwords per strip row
   uint32_t last_even;              // 1: the last table's modulus is even (a power of 2): its
                                    // rows map x -> x (-M)^-1 (one row per M subtracted)
+  uint32_t word_bytes;             // bytes from a mask's word-w table to its word-(w+1) table
   const uint32_t* lds_src;         // device copy of the LDS image
   const uint32_t* perm;            // [G * 64] strip row of each lane slot; bit 31: a padding
                                    // lane (it repeats a row of its half-wave and emits nothing)
@@ -549,9 +560,9 @@ __device__ __forceinline__ void es_rows_split(uint32_t q0, uint32_t Hp, uint64_t
 }
 
 // Exact re-test of queued entries [first, first + n) (one per lane), called by the whole wave;
-// an entry is {bits, v | lane << 26}: the entry's lane supplies its sums, strip row and
-// correction through cross-lane permutes, its step row is read by vector loads.
-template <int C>
+// an entry is {bits, v | w << 10 | lane << 26}: the entry's lane supplies its sums, strip row
+// and correction through cross-lane permutes, its step row is read by vector loads.
+template <int C, int W>
 __device__ __forceinline__ void es_rows_drain(const RowPlan& P, const uint2* q, uint32_t first,
                                               uint32_t n, uint32_t Hp, uint64_t Lpl, uint64_t Lph,
                                               uint32_t sg, uint32_t X, uint64_t obase,
@@ -566,7 +577,8 @@ __device__ __forceinline__ void es_rows_drain(const RowPlan& P, const uint2* q, 
                        ((uint64_t)es_from_lane(src, (uint32_t)(Lph >> 32)) << 32);
   const uint32_t sH = es_from_lane(src, Hp), ssg = es_from_lane(src, sg), sX = es_from_lane(src, X);
   if (lane < n) {
-    const uint32_t v = ent.y & ((1u << kQueueIndexBits) - 1u);
+    const uint32_t v = ent.y & ((1u << kRowsWordShift) - 1u);
+    const uint32_t w = W == 1 ? 0u : (ent.y >> kRowsWordShift) & (kRowsMaxWords - 1u);
     const uint32_t* st = P.steps + (size_t)v * kStepDwords;
     const uint64_t zl = (uint64_t)st[1] | ((uint64_t)st[2] << 32);
     const uint64_t zh = (uint64_t)st[3] | ((uint64_t)st[4] << 32);
@@ -578,13 +590,14 @@ __device__ __forceinline__ void es_rows_drain(const RowPlan& P, const uint2* q, 
     const u128 l = kap ? Lp - z : Lp + (M0 - z);   // Lp + l_D (mod M0)
     const uint64_t o = obase + (uint64_t)v * P.S + ssg;
     const uint64_t ob = (C & 2) ? o * P.r0 + (uint64_t)(int64_t)(int32_t)sX : o;
-    es_exact_test_hl<C>(P.cold, ent.x, 0, h, l, ob, idx_lo, idx_hi);
+    es_exact_test_hl<C>(P.cold, ent.x, w, h, l, ob, idx_lo, idx_hi);
   }
   __builtin_amdgcn_wave_barrier();
 }
 
-// One unit: lane group g of the strip rows, superstrip w, all V steps.  Wave-uniform.
-template <int F, int C, bool CHECK>
+// One unit: lane group g of the strip rows, superstrip w, all V steps.  Wave-uniform.  W: words
+// per ring-0 row (W >= 2 reads the byte table of j(h), C == 1).
+template <int F, int C, bool CHECK, int W>
 __device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* lds,
                                               const uint32_t (&A)[F + 1], uint32_t Hp,
                                               uint64_t Lpl, uint64_t Lph, uint32_t sg,
@@ -599,6 +612,9 @@ __device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* 
   uint2* wq = es_rows_queue(P, lds);
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t vmask = __ballot(valid);
+  static_assert(W >= 1 && W <= kRowsMaxWords && (W == 1 || C == 1),
+                "wider rows read the byte carry table and have no lead block");
+  const uint32_t wb = W > 1 ? P.word_bytes : 0u;
   uint32_t cnt = 0;
   for (uint32_t v = 0; v < V; ++v) {
     cu32_t* st = (cu32_t*)(P.steps) + (size_t)v * kStepDwords;
@@ -609,46 +625,65 @@ __device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* 
     es_rows_split(q0, Hp, Lpl, Lph, st[0], zl, zh, h, c, kap);
     const uint32_t wrap = c ? 256u : 0u;      // 32 rows of 8 bytes when M was subtracted
     const uint32_t wrap_last = c ? (P.last_even ? 8u : 256u) : 0u;
-    uint32_t bx = 0xffffffffu, by = 0xffffffffu;
+    uint32_t bx[W], by[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) bx[w] = by[w] = 0xffffffffu;
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      const uint2 x = *reinterpret_cast<const uint2*>(lb + A[f] + st[5 + f] +
-                                                      (f == F - 1 ? wrap_last : wrap));
-      bx &= x.x;
-      by &= x.y;
-    }
-    uint32_t carry;
-    if (C & 1) {  // byte table of j(h) <= 32: bits i >= j carry (one 64-bit shift)
-      carry = (uint32_t)(0xffffffffull << carry_tab[h]);
-    } else {
-      carry = carry_w[h];
-    }
-    uint32_t bits = (carry & by) | (~carry & bx);
-    if (CHECK) {
-      const uint64_t o = obase + (uint64_t)v * S + sg;
-      if (o < o_lo || o >= o_hi) bits = 0;
-    }
-    // padding lanes (a repeated row) are left out through the wave's mask of valid lanes
-    const uint64_t any = __ballot(bits != 0u) & vmask;
-    if (any) {
-      if (bits && valid) {
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
-        wq[cnt + below] = make_uint2(bits, v | (lane << kQueueIndexBits));
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint2 x = *reinterpret_cast<const uint2*>(
+            lb + A[f] + st[5 + f] + (f == F - 1 ? wrap_last : wrap) + (uint32_t)w * wb);
+        bx[w] &= x.x;
+        by[w] &= x.y;
       }
-      cnt += (uint32_t)__popcll(any);
-      if (cnt >= 64u) {
-        cnt -= 64u;
-        es_rows_drain<C>(P, wq, cnt, 64u, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+    }
+    // bits i >= j(h) of the ring-0 row carry past M
+    uint32_t carry[W];
+    if constexpr (W == 1) {
+      if (C & 1) {  // byte table of j(h) <= 32: one 64-bit shift
+        carry[0] = (uint32_t)(0xffffffffull << carry_tab[h]);
+      } else {
+        carry[0] = carry_w[h];
+      }
+    } else {        // byte table of j(h) <= 64: word w carries from bit j - 32 w on
+      const int j = (int)carry_tab[h];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int sh = j - 32 * w;
+        carry[w] = sh <= 0 ? 0xffffffffu : sh >= 32 ? 0u : 0xffffffffu << sh;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint32_t bits = (carry[w] & by[w]) | (~carry[w] & bx[w]);
+      if (CHECK) {
+        const uint64_t o = obase + (uint64_t)v * S + sg;
+        if (o < o_lo || o >= o_hi) bits = 0;
+      }
+      // padding lanes (a repeated row) are left out through the wave's mask of valid lanes
+      const uint64_t any = __ballot(bits != 0u) & vmask;
+      if (any) {
+        if (bits && valid) {
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
+          wq[cnt + below] = make_uint2(
+              bits, v | ((uint32_t)w << kRowsWordShift) | (lane << kQueueIndexBits));
+        }
+        cnt += (uint32_t)__popcll(any);
+        if (cnt >= 64u) {  // at most 127 queued: a drain follows every word's append
+          cnt -= 64u;
+          es_rows_drain<C, W>(P, wq, cnt, 64u, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+        }
       }
     }
   }
-  if (cnt) es_rows_drain<C>(P, wq, 0, cnt, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+  if (cnt) es_rows_drain<C, W>(P, wq, 0, cnt, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
 }
 
 // Unit set-up: decode superstrip w, its table rows (block constant included), the lane's
 // strip row and sums; then sweep the V steps.
-template <int F, int C>
+template <int F, int C, int W>
 __device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* lds, uint32_t g,
                                              uint64_t w, uint64_t o_lo, uint64_t o_hi,
                                              uint64_t idx_lo, uint64_t idx_hi) {
@@ -738,17 +773,17 @@ __device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* l
   const uint64_t obase = w * span;
   const bool full = obase >= o_lo && obase + span <= o_hi;
   if (full)
-    es_rows_sweep<F, C, false>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo, o_hi,
-                               idx_lo, idx_hi);
+    es_rows_sweep<F, C, false, W>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo,
+                                  o_hi, idx_lo, idx_hi);
   else
-    es_rows_sweep<F, C, true>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo, o_hi,
-                              idx_lo, idx_hi);
+    es_rows_sweep<F, C, true, W>(P, lds, A, Hp, Lpl, Lph, sg, srow[1], valid, obase, o_lo,
+                                 o_hi, idx_lo, idx_hi);
 }
 
 // Units g + G * (w - w_lo) for w in [w_lo, w_hi), one per wave: the grid covers them all (the
 // host splits a range whose units exceed one grid).
-template <int F, int C>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+template <int F, int C, int W>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(W == 1 ? 6 : 5)))
 es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uint64_t o_lo,
                uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
   const RowPlan& P = *Pp;
@@ -761,14 +796,15 @@ es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uin
   const uint64_t q = (uint64_t)blockIdx.x * wpg + wave;
   if (q < units) {
     const uint64_t wq = q / P.G;
-    es_rows_unit<F, C>(P, lds, (uint32_t)(q - wq * P.G), w_lo + wq, o_lo, o_hi, idx_lo, idx_hi);
+    es_rows_unit<F, C, W>(P, lds, (uint32_t)(q - wq * P.G), w_lo + wq, o_lo, o_hi, idx_lo,
+                          idx_hi);
   }
 }
 
 // Grid-mapped batch of row-lane jobs: workgroup b serves units (b - wg_start) * waves + wave
 // of the job whose workgroup range holds b.  GridJob's u_lo/u_hi hold the job's superstrips.
-template <int F, int C>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+template <int F, int C, int W>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(W == 1 ? 6 : 5)))
 es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
                      uint32_t n_jobs) {
   extern __shared__ uint32_t lds[];
@@ -787,34 +823,38 @@ es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restric
   const uint64_t q = (b - J.wg_start) * wpg + wave;
   if (q < (uint64_t)P.G * (J.u_hi - J.u_lo)) {
     const uint64_t wq = q / P.G;
-    es_rows_unit<F, C>(P, lds, (uint32_t)(q - wq * P.G), J.u_lo + wq, J.o_lo, J.o_hi, J.idx_lo,
-                       J.idx_hi);
+    es_rows_unit<F, C, W>(P, lds, (uint32_t)(q - wq * P.G), J.u_lo + wq, J.o_lo, J.o_hi,
+                          J.idx_lo, J.idx_hi);
   }
 }
 
 typedef void (*RowsFn)(const RowPlan*, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
 typedef void (*RowsBatchFn)(const RowPlan*, const GridJob*, uint32_t);
 
+// Row-lane instances: W = 1 with each carry form and lead-block form (C = 0..3); W = 2 with
+// the byte carry table (C = 1).
 template <int F>
-RowsFn rows_kernel_for(uint32_t C) {
+RowsFn rows_kernel_for(uint32_t C, uint32_t W) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  if (W != 1) return W == 2 && C == 1 ? es_rows_kernel<F, 1, 2> : nullptr;
   switch (C) {
-    case 0: return es_rows_kernel<F, 0>;
-    case 1: return es_rows_kernel<F, 1>;
-    case 2: return es_rows_kernel<F, 2>;
-    case 3: return es_rows_kernel<F, 3>;
+    case 0: return es_rows_kernel<F, 0, 1>;
+    case 1: return es_rows_kernel<F, 1, 1>;
+    case 2: return es_rows_kernel<F, 2, 1>;
+    case 3: return es_rows_kernel<F, 3, 1>;
     default: return nullptr;
   }
 }
 
 template <int F>
-RowsBatchFn rows_batch_for(uint32_t C) {
+RowsBatchFn rows_batch_for(uint32_t C, uint32_t W) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
+  if (W != 1) return W == 2 && C == 1 ? es_rows_batch_kernel<F, 1, 2> : nullptr;
   switch (C) {
-    case 0: return es_rows_batch_kernel<F, 0>;
-    case 1: return es_rows_batch_kernel<F, 1>;
-    case 2: return es_rows_batch_kernel<F, 2>;
-    case 3: return es_rows_batch_kernel<F, 3>;
+    case 0: return es_rows_batch_kernel<F, 0, 1>;
+    case 1: return es_rows_batch_kernel<F, 1, 1>;
+    case 2: return es_rows_batch_kernel<F, 2, 1>;
+    case 3: return es_rows_batch_kernel<F, 3, 1>;
     default: return nullptr;
   }
 }
@@ -871,14 +911,14 @@ GridBatchFn pick_grid_batch_p0(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p1(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p2(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p3(int F, uint32_t W, uint32_t G, uint32_t C);
-RowsFn pick_rows_p0(int F, uint32_t C);
-RowsFn pick_rows_p1(int F, uint32_t C);
-RowsFn pick_rows_p2(int F, uint32_t C);
-RowsFn pick_rows_p3(int F, uint32_t C);
-RowsBatchFn pick_rows_batch_p0(int F, uint32_t C);
-RowsBatchFn pick_rows_batch_p1(int F, uint32_t C);
-RowsBatchFn pick_rows_batch_p2(int F, uint32_t C);
-RowsBatchFn pick_rows_batch_p3(int F, uint32_t C);
+RowsFn pick_rows_p0(int F, uint32_t C, uint32_t W);
+RowsFn pick_rows_p1(int F, uint32_t C, uint32_t W);
+RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W);
+RowsFn pick_rows_p3(int F, uint32_t C, uint32_t W);
+RowsBatchFn pick_rows_batch_p0(int F, uint32_t C, uint32_t W);
+RowsBatchFn pick_rows_batch_p1(int F, uint32_t C, uint32_t W);
+RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W);
+RowsBatchFn pick_rows_batch_p3(int F, uint32_t C, uint32_t W);
 
 // Mask counts served by each instance unit.
 constexpr int kPartFirstF[5] = {0, 4, 7, 10, 13};

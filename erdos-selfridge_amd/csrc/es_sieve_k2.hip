@@ -22,20 +22,20 @@ GridBatchFn pick_grid_batch_p2(int F, uint32_t W, uint32_t G, uint32_t C) {
   }
 }
 
-RowsFn pick_rows_p2(int F, uint32_t C) {
+RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W) {
   switch (F) {
-    case 7: return rows_kernel_for<7>(C);
-    case 8: return rows_kernel_for<8>(C);
-    case 9: return rows_kernel_for<9>(C);
+    case 7: return rows_kernel_for<7>(C, W);
+    case 8: return rows_kernel_for<8>(C, W);
+    case 9: return rows_kernel_for<9>(C, W);
     default: return nullptr;
   }
 }
 
-RowsBatchFn pick_rows_batch_p2(int F, uint32_t C) {
+RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W) {
   switch (F) {
-    case 7: return rows_batch_for<7>(C);
-    case 8: return rows_batch_for<8>(C);
-    case 9: return rows_batch_for<9>(C);
+    case 7: return rows_batch_for<7>(C, W);
+    case 8: return rows_batch_for<8>(C, W);
+    case 9: return rows_batch_for<9>(C, W);
     default: return nullptr;
   }
 }

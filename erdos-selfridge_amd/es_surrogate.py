@@ -458,9 +458,11 @@ def surrogate_plan(k: int, model: Surrogate | None, n_candidates: int = 100_000,
     """Rank n_candidates wheels by log2 E_pos + predicted log2 ns, probe the first `shortlist`
     and the plain wheel, and return the least E_pos * measured ns (the plain wheel when no
     shortlisted wheel probes).  model=None ranks by log2 E_pos alone (the analytic-only
-    shortlist).  `compare` lists shortlist sizes at which both rankings are evaluated from one
-    shared pool of probes ("sizes" in the result: best cost of each ranking's first s wheels
-    plus the plain wheel, and the first wheel of each ranking with no fallback)."""
+    shortlist).  `compare` lists probe budgets s at which both rankings are evaluated from
+    one shared pool of probes ("sizes" in the result: the least cost among the plain wheel and
+    each ranking's first s wheels other than the plain wheel -- s probes besides the plain
+    wheel for either ranking, whose analytic first wheel is the plain one -- and the first
+    wheel of each ranking with no fallback)."""
     t0 = time.perf_counter()
     sp = PlanSpace(k)
     E = sp.enumerate(n_candidates, seed=seed)
@@ -487,13 +489,16 @@ def surrogate_plan(k: int, model: Surrogate | None, n_candidates: int = 100_000,
         got = [r for r in (probed(i) for i in list(idx) + [plain_i]) if r is not None]
         return min(got, key=lambda r: r["cost_s"])
 
+    def others(r, s_):
+        return [i for i in r[:s_ + 1] if i != plain_i][:s_]
+
     best = best_of(rank[:shortlist])
     plain_r = probed(plain_i)
     sizes = {}
     for s_ in compare:
-        row = {"analytic": best_of(rank_a[:s_])["cost_s"] / plain_r["cost_s"]}
+        row = {"analytic": best_of(others(rank_a, s_))["cost_s"] / plain_r["cost_s"]}
         if rank_m is not None:
-            row["surrogate"] = best_of(rank_m[:s_])["cost_s"] / plain_r["cost_s"]
+            row["surrogate"] = best_of(others(rank_m, s_))["cost_s"] / plain_r["cost_s"]
         sizes[int(s_)] = row
     if compare and rank_m is not None:
         top = probed(rank_m[0])

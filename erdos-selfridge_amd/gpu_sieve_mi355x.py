@@ -166,11 +166,13 @@ def engine_geometry(radix, lead_rings: int = 0, strip_target: int = 1024):
 
 
 def rows_layout(radix, lead_rings: int, tun: dict) -> bool:
-    """Whether the engine sweeps this ring order in the row-lane layout (one-word rows, every
-    mask in LDS, a strip of >= 32 rows and >= 16 steps, not forced to strips)."""
-    if str(tun.get("mi_layout", "auto")) == "strips" or int(tun.get("mi_global_masks", 0)):
+    """Whether the engine sweeps this ring order in the row-lane layout (every mask in LDS, a
+    strip of >= 32 rows and >= 16 steps; one-word rows under "auto", two-word rows too under
+    "rows")."""
+    mode = str(tun.get("mi_layout", "auto"))
+    if mode == "strips" or int(tun.get("mi_global_masks", 0)):
         return False
-    if not radix or radix[0] > 32:
+    if not radix or radix[0] > (64 if mode == "rows" else 32):
         return False
     S, _, V = engine_geometry(radix, lead_rings, int(tun.get("mi_strip_target", 1024)))
     return S >= 32 and V >= 16
@@ -187,8 +189,8 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     With `stop` > 0, masks are added only while the expected candidate slots per outer index
     (r0 x the pass rates taken) is at least `stop`: below that, a further mask costs more LDS
     reads than the queued exact re-test it saves.  rows: the row-lane layout's tables, rows
-    [0, 3m + 64) of one 8-byte row per ring-0 word, instead of [0, 2m) of two words.
-    Returns [(filter_index, D, modulus, conditional pass rate)]."""
+    [0, 3m + 64) of one 8-byte row per ring-0 word (a table per word), instead of [0, 2m) of
+    two words per ring-0 word.  Returns [(filter_index, D, modulus, conditional pass rate)]."""
     k = plan.k
     rest = plan.rest if limit is None else plan.rest[:limit]
     if r0 is None:
@@ -208,12 +210,16 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
             rate = (partial_count(k, p, d) / p ** d) / base
             cands.append((rate, fi, d, p ** d))
     cands.sort(key=lambda c: (c[0], c[1], -c[2]))
-    # carry table: q0 rows of `words` words, or q0 u16 suffix starts when words > 2
-    carry = (q0 + 1) // 2 if words > 2 else q0 * words
+    # carry table: q0 rows of `words` words, or q0 u16 suffix starts when words > 2; row-lane
+    # rows of several words: q0 bytes
+    if rows and words > 1:
+        carry = (q0 + 3) // 4
+    else:
+        carry = (q0 + 1) // 2 if words > 2 else q0 * words
     room = lds_budget // 4 - carry - 8
 
     def cost_of(m):
-        return (2 * (3 * m + 64) + 4) if rows else (4 * words * m + 4)
+        return (2 * words * (3 * m + 64) + 4) if rows else (4 * words * m + 4)
 
     def greedy(left):
         taken, out = set(), []
@@ -234,27 +240,27 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
     if stop > 0 and out != greedy(1 << 62):
         # The budget cut the greedy short (a large table taken first can crowd out several
         # small ones): choose within the budget by the kernel's cost per outer index instead.
-        alt = _masks_within_budget(cands, n_masks, room, cost_of, r0)
-        if _mask_cost(alt, r0) < _mask_cost(out, r0):
+        alt = _masks_within_budget(cands, n_masks, room, cost_of, r0, words)
+        if _mask_cost(alt, r0, words) < _mask_cost(out, r0, words):
             out = alt
     return out
 
 
-# Cost of a mask set per outer index, in mask-row reads: one read per mask and the carry read,
-# plus ~160 reads' worth of work per candidate slot queued for the exact re-test (row-lane
-# counters: k=301 with 0.041 slots per outer index runs its wave-steps 1.67x slower than k=304
-# with 0.009, at the same six reads per step).
+# Cost of a mask set per outer index, in mask-row reads: one read per mask and ring-0 word and
+# the carry read, plus ~160 reads' worth of work per candidate slot queued for the exact
+# re-test (row-lane counters: k=301 with 0.041 slots per outer index runs its wave-steps 1.67x
+# slower than k=304 with 0.009, at the same six reads per step).
 _EXACT_PER_SLOT = 160.0
 
 
-def _mask_cost(masks, r0: int) -> float:
+def _mask_cost(masks, r0: int, words: int = 1) -> float:
     dens = float(r0)
     for _, _, _, rate in masks:
         dens *= rate
-    return len(masks) + 1 + _EXACT_PER_SLOT * dens
+    return words * len(masks) + 1 + _EXACT_PER_SLOT * dens
 
 
-def _masks_within_budget(cands, n_masks: int, room: int, cost_of, r0: int):
+def _masks_within_budget(cands, n_masks: int, room: int, cost_of, r0: int, words: int = 1):
     """Least-cost mask set (one depth per prime, at most n_masks, LDS words within room): a
     knapsack over the primes' depth options in units of 64 words, minimising the sum of log
     pass rates for each (count, units), then the cost of each reachable state."""
@@ -283,7 +289,7 @@ def _masks_within_budget(cands, n_masks: int, room: int, cost_of, r0: int):
         best = nb
     # cost of every reachable (count, units) state; walk the choices back from the best one
     cnt = np.arange(n_masks + 1)[:, None]
-    cost = cnt + 1 + _EXACT_PER_SLOT * r0 * np.exp(np.minimum(best, 0.0))
+    cost = words * cnt + 1 + _EXACT_PER_SLOT * r0 * np.exp(np.minimum(best, 0.0))
     cost[~np.isfinite(best)] = inf
     c, u = np.unravel_index(int(np.argmin(cost)), cost.shape)
     out = []
@@ -514,9 +520,9 @@ class GpuSieve:
         P = plan
         self._h = None
         L = nat.lib()
-        self.inner_bound, self.ring_swap = self._choose_layout(P)
+        self.inner_bound, self.ring_swap, self.lane_layout = self._choose_layout(P)
         self._d = _Desc(P, dict(P.tun, mi_inner_radix=self.inner_bound,
-                                mi_ring_swap=self.ring_swap))
+                                mi_ring_swap=self.ring_swap, mi_layout=self.lane_layout))
         h = C.c_void_p()
         nat.check(L.es_engine_create(C.byref(self._d.desc), C.byref(h)))
         self._h = h
@@ -559,14 +565,18 @@ class GpuSieve:
             self._h = None
 
     @staticmethod
-    def _layout_options(P: Plan) -> list[tuple[int, object]]:
-        """Distinct kernel ring layouts of a plan as (mi_inner_radix, mi_ring_swap) pairs:
+    def _layout_options(P: Plan) -> list[tuple[int, object, str]]:
+        """Distinct kernel layouts of a plan as (mi_inner_radix, mi_ring_swap, mi_layout):
         the digit-split bounds that give distinct kernel inner radices (a ring 0 wider than 64
-        slots mod p^T, T >= 2: whole ring, groups <= 128, 64, 32, 16), and for each layout
-        whose ring 0 stays whole and could trade places with a later ring of <= 32 residues,
-        both orders.  Tunables other than "auto" fix their dimension."""
+        slots mod p^T, T >= 2: whole ring, groups <= 128, 64, 32, 16); for each whose ring 0
+        stays whole and could trade places with a later ring of <= 32 residues, both orders;
+        and where the kernel's ring 0 then has two words (33..64 slots), both lane layouts
+        (strip-lane, the "auto" default there, and row-lane).  Tunables other than "auto" fix
+        their dimension."""
         inner = P.tun.get("mi_inner_radix", "auto")
         swap = swap_mode(P.tun)
+        layout = str(P.tun.get("mi_layout", "auto"))
+        glob = int(P.tun.get("mi_global_masks", 0))
         if inner != "auto":
             bounds = [int(inner)]
         elif not P.rings or P.radix[0] <= 64 or P.wheel[0][1] < 2:
@@ -579,20 +589,28 @@ class GpuSieve:
             if radix[0] in seen:
                 continue
             seen.add(radix[0])
-            can = (not int(P.tun.get("mi_global_masks", 0)) and
-                   swap_lead(radix, inner_mod, force=True) > 0)
+            can = not glob and swap_lead(radix, inner_mod, force=True) > 0
             if swap != "auto" or not can:
-                opts.append((bound, swap if swap != "auto" else 0))
+                swaps = [swap if swap != "auto" else 0]
             else:
-                opts += [(bound, 0), (bound, 1)]
+                swaps = [0, 1]
+            for sw in swaps:
+                j = 0 if glob or sw == 0 else swap_lead(radix, inner_mod, force=sw == 1)
+                r0k = radix[j]
+                if layout == "auto" and not glob and 32 < r0k <= 64:
+                    opts += [(bound, sw, "auto"), (bound, sw, "rows")]
+                else:
+                    opts.append((bound, sw, layout))
         return opts
 
     @staticmethod
-    def _choose_layout(P: Plan, strips_log2: int = 19) -> tuple[int, object]:
-        """(mi_inner_radix, mi_ring_swap) for the engine: with one layout option, that one;
-        otherwise each option is timed on a window of block 1 and the fastest kept.  Every
-        option scans the same positions with the same exact test; only the LDS layout (words
-        per row, masks that fit, strip-lane or row-lane lanes) and so the speed differ.  Each
+    def _choose_layout(P: Plan, strips_log2: int = 19) -> tuple[int, object, str]:
+        """(mi_inner_radix, mi_ring_swap, mi_layout) for the engine: with one layout option,
+        that one; otherwise each option is timed on a window of block 1 and the fastest kept.
+        Every option scans the same positions with the same exact test; only the LDS layout
+        (words per row, masks that fit, strip-lane or row-lane lanes) and so the speed differ.
+        An option the library refuses (a row-lane layout without the strip or steps it needs)
+        is skipped.  Each
         window holds 2^strips_log2 strips of its own layout (about 8 workgroups per CU, so long
         strips are not timed on an under-filled GPU), between 2^28 and 2^34 positions."""
         opts = GpuSieve._layout_options(P)
@@ -600,9 +618,9 @@ class GpuSieve:
             return opts[0]
         L = nat.lib()
         best, best_rate = opts[0], 0.0
-        for bound, swap in opts:
+        for bound, swap, lay in opts:
             try:
-                d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap))
+                d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap, mi_layout=lay))
             except AssertionError:
                 continue
             h = C.c_void_p()
@@ -624,7 +642,7 @@ class GpuSieve:
                 rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05,
                                   floor=0.5 * best_rate)
                 if rate > best_rate:
-                    best, best_rate = (bound, swap), rate
+                    best, best_rate = (bound, swap, lay), rate
             finally:
                 L.es_engine_destroy(h)
         return best
@@ -734,12 +752,13 @@ class GpuSieve:
         """Algorithmic LDS bytes the sieve kernel reads per residue position: per outer index
         (r0 positions of the kernel's ring 0) each lane reads one mask row of 8 bytes per
         ring-0 word from each of the masks staged in LDS, and its carry entry -- one byte of the
-        j(h) table (one-word rows, q0 <= 256, every mask in LDS), a row of one 4-byte word per
-        ring-0 word (two words), or a u16 j(h) (wider rows).  The candidate queue (a few
-        entries per thousand outer indices) and the per-workgroup table staging are left out."""
+        j(h) table (one-word rows, q0 <= 256, every mask in LDS; every row-lane row of two to
+        four words), a row of one 4-byte word per ring-0 word (two words), or a u16 j(h)
+        (wider rows).  The candidate queue (a few entries per thousand outer indices) and the
+        per-workgroup table staging are left out."""
         info = self.info
         W, F, G = int(info.words), int(info.n_masks_used), int(info.global_masks)
-        if W == 1 and info.q0 <= 256 and G == 0:
+        if (W == 1 and info.q0 <= 256 and G == 0) or (W > 1 and info.layout == 2):
             carry = 1
         elif W <= 2:
             carry = 4 * W

@@ -94,8 +94,9 @@ typedef struct es_plan_desc {
                                   ring_radix[1..j] of a lead block (see above) */
   uint32_t layout;             /* 0: row-lane layout where it applies (one-word rows, every
                                   mask in LDS, at most 11 masks), else strip-lane; 1: always
-                                  strip-lane; 2: row-lane or ES_ERR_LIMIT.  Speed only: both
-                                  layouts return the same survivors. */
+                                  strip-lane; 2: row-lane, also for two-word rows (a ring 0 of
+                                  33..64 slots), or ES_ERR_LIMIT.  Speed only: both layouts
+                                  return the same survivors. */
 } es_plan_desc;
 
 typedef struct es_engine_info {

@@ -156,7 +156,8 @@ def _recorded_runs():
         for role in ("primary", "confirmation"):
             P = EP.Plan(t["k"], None, t[role]["wheel"])
             opts = G.GpuSieve._layout_options(P)
-            swaps = sorted({s for _, s in opts}) if len(opts) > 1 else ["auto"]
+            swaps = sorted({s for _, s, _ in opts})
+            swaps = swaps if len(swaps) > 1 else ["auto"]
             out += [(t["k"], role, s) for s in swaps]
     return out
 
@@ -464,3 +465,55 @@ def test_row_lane_layout_equals_strip_lane(k, wheel):
         assert sorted(a) == [int(x) for x in want] and amin == wmin
     if rows.info.words == 1 and rows.strip_len >= 32:
         assert rows.layout == "rows"
+
+
+@pytest.mark.parametrize("k,wheel", [(392, None), (384, None), (328, None), (385, None),
+                                     (399, K399_CONFIRM), (383, None)])
+def test_row_lane_wider_rows(k, wheel):
+    """Rings 0 of 33..64 residues kept whole (mi_inner_radix 128: no digit rings) in the plan's
+    order (mi_ring_swap 0) run the row-lane layout with two words per ring-0 row (k=384: 49
+    residues, k=392 and 328: 64, the k=399 confirmation wheel: 47, k=383: 56, k=385: 42), one
+    table per word: the same survivors and minimum as the strip-lane layout and the oracle on
+    dense filter-prefix windows at unaligned edges, over several launches, and for an
+    outer-unit launch sequence.  A three-word ring 0 asked for row-lane is refused."""
+    import gpu_sieve_mi355x as G
+    rng = random.Random(5 * k + 1)
+    plan = EP.Plan(k, None, wheel)
+    tune = {"mi_inner_radix": 128}
+    for limit in (3, 6, None):
+        base = dict(tune, max_survivors=1 << 22, mi_launch_log2=26, mi_ring_swap=0)
+        if limit is not None:
+            base["mi_filter_limit"] = limit
+        rows = G.GpuSieve(EP.Plan(k, dict(base, mi_layout="rows"), wheel))
+        strips = G.GpuSieve(EP.Plan(k, dict(base, mi_layout="strips"), wheel))
+        assert rows.layout == "rows" and strips.layout == "strips" and rows._d.lead == 0
+        assert rows.info.words == (plan.radix[0] + 31) // 32 >= 2
+        span = min(1 << 24, plan.R // 3)
+        t = rng.randrange(0, 1 << 16)
+        lo = rng.randrange(0, plan.R - span)
+        hi = lo + span - rng.randrange(0, 1000)
+        say(f"case wide rows k={k} W={rows.info.words} limit={limit} t={t} window=[{lo},{hi}) "
+            f"masks={rows.info.n_masks_used} steps={rows.info.steps} groups={rows.info.groups}")
+        a, amin = rows.scan_window(t, lo, hi)
+        b, bmin = strips.scan_window(t, lo, hi)
+        assert sorted(a) == sorted(b) and amin == bmin
+        filt = plan.rest if limit is None else plan.rest[:limit]
+        n, want, wmin = O.c_scan_min(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
+        assert sorted(a) == [int(x) for x in want] and amin == wmin
+    # outer-unit launches (the drivers' _launch) over a dense prefix
+    eng = G.GpuSieve(EP.Plan(k, dict(tune, mi_filter_limit=4, max_survivors=1 << 22,
+                                     mi_ring_swap=0, mi_layout="rows"), wheel))
+    eng.set_block(3)
+    eng.out_n.fill(0)
+    o0 = plan.R // eng.r0 // 5
+    eng._launch(o0, 30001)
+    eng._launch(o0 + 30001, 7)
+    n = int(eng.out_n.get()[0])
+    got = np.sort(np.asarray([int(x) for x in eng.out[:n].get()], dtype=np.uint64))
+    lo, hi = o0 * eng.r0, (o0 + 30008) * eng.r0
+    nw, want = O.c_scan(k, plan.wheel, plan.rest[:4], 3, lo, hi, threads=16, cap=1 << 22)
+    assert n == nw and np.array_equal(got, want)
+    if k == 392:
+        with pytest.raises(Exception, match="two words"):
+            G.GpuSieve(EP.Plan(400, {"mi_inner_radix": 128, "mi_ring_swap": 0,
+                                     "mi_layout": "rows"}))

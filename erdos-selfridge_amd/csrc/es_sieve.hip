@@ -92,25 +92,46 @@ RowsBatchFn pick_rows_batch(int F, uint32_t C, uint32_t W) {
 // table f; a half-wave costs, per table, its most crowded bank pair (the cycles one
 // ds_read_b64 of those 32 lanes takes).  Rows are placed greedily (each to the open half-wave
 // it crowds least, in a seeded shuffled order), then pairs of slots in different half-waves
-// are swapped while the swap does not raise the two half-waves' cost.  Empty slots repeat a
-// row of their half-wave (the same address: no extra bank load) and carry bit 31 (no output).
+// are swapped while the swap does not raise the two half-waves' cost.  The first slot of a
+// pair is drawn from a half-wave above the floor of two lanes on the most crowded pair of
+// every table (~31 live rows over 32 pairs rarely allow fewer), on a row that sits on such a
+// pair; the search stops once no half-wave is above it.  (Random pairs, the earlier search,
+// left wheels with long strips at 4.4-4.5 cycles per read where this reaches 4.0 in the same
+// number of swaps.)  Empty slots repeat a row of their half-wave (the same address: no extra
+// bank load) and carry bit 31 (no output).
 std::vector<uint32_t> group_strip_rows(const std::vector<uint32_t>& bank, uint32_t S, uint32_t F,
                                        uint32_t G) {
   const uint32_t H = 2 * G;                      // half-waves
   const uint32_t FF = std::max<uint32_t>(F, 1);
-  std::vector<uint8_t> cnt((size_t)H * FF * 32, 0);
+  // per (half-wave, table): bank-pair loads, the largest load and how many pairs carry it;
+  // per half-wave: its cost, the sum of the largest loads over the tables
+  std::vector<uint8_t> cnt((size_t)H * FF * 32, 0), mx((size_t)H * FF, 0), nmx((size_t)H * FF, 32);
+  std::vector<uint32_t> hcost(H, 0);
   std::vector<uint32_t> slot((size_t)H * 32, UINT32_MAX), fill(H, 0);
   auto bin = [&](uint32_t h, uint32_t f, uint32_t b) -> uint8_t& {
     return cnt[((size_t)h * FF + f) * 32 + b];
   };
-  auto cost = [&](uint32_t h) {
-    uint32_t c = 0;
-    for (uint32_t f = 0; f < F; ++f) {
-      uint8_t mx = 0;
-      for (uint32_t b = 0; b < 32; ++b) mx = std::max(mx, bin(h, f, b));
-      c += mx;
+  auto inc = [&](uint32_t h, uint32_t f, uint32_t b) {
+    const size_t k = (size_t)h * FF + f;
+    const uint8_t c = ++bin(h, f, b);
+    if (c > mx[k]) {
+      mx[k] = c;
+      nmx[k] = 1;
+      ++hcost[h];
+    } else if (c == mx[k]) {
+      ++nmx[k];
     }
-    return c;
+  };
+  auto dec = [&](uint32_t h, uint32_t f, uint32_t b) {
+    const size_t k = (size_t)h * FF + f;
+    const uint8_t c = bin(h, f, b)--;
+    if (c == mx[k] && --nmx[k] == 0) {   // the only most crowded pair lost a lane: recount
+      mx[k] = c - 1;
+      --hcost[h];
+      uint8_t n = 0;
+      for (uint32_t x = 0; x < 32; ++x) n += bin(h, f, x) == mx[k];
+      nmx[k] = n;
+    }
   };
   uint64_t rng = 0x9e3779b97f4a7c15ull;
   auto next = [&]() {
@@ -127,40 +148,75 @@ std::vector<uint32_t> group_strip_rows(const std::vector<uint32_t>& bank, uint32
     for (uint32_t h = 0; h < H && best_add; ++h) {
       if (fill[h] >= 32) continue;
       uint32_t add = 0;
-      for (uint32_t f = 0; f < F; ++f) {
-        const uint8_t c = bin(h, f, bank[(size_t)r * F + f]);
-        uint8_t mx = 0;
-        for (uint32_t b = 0; b < 32; ++b) mx = std::max(mx, bin(h, f, b));
-        add += c + 1u > mx ? 1u : 0u;
-      }
+      for (uint32_t f = 0; f < F; ++f)
+        add += bin(h, f, bank[(size_t)r * F + f]) + 1u > mx[(size_t)h * FF + f] ? 1u : 0u;
       if (add < best_add) {
         best_add = add;
         best_h = h;
       }
     }
     slot[(size_t)best_h * 32 + fill[best_h]++] = r;
-    for (uint32_t f = 0; f < F; ++f) ++bin(best_h, f, bank[(size_t)r * F + f]);
+    for (uint32_t f = 0; f < F; ++f) inc(best_h, f, bank[(size_t)r * F + f]);
   }
-  const uint64_t iters = std::min<uint64_t>(60000, (uint64_t)S * 60);
+  auto move = [&](uint32_t r, uint32_t from, uint32_t to) {
+    if (r == UINT32_MAX) return;
+    for (uint32_t f = 0; f < F; ++f) {
+      dec(from, f, bank[(size_t)r * F + f]);
+      inc(to, f, bank[(size_t)r * F + f]);
+    }
+  };
+  const uint32_t floor_cost = 2 * F;
+  auto above_floor = [&]() {
+    uint32_t n = 0;
+    for (uint32_t h = 0; h < H; ++h) n += hcost[h] > floor_cost;
+    return n;
+  };
+  // swaps: at most 120 per strip row (200,000 in all); stop once every half-wave is at the
+  // floor, or after 20,000 swaps without a lower total cost
+  const uint64_t iters = std::min<uint64_t>(200000, (uint64_t)S * 120);
   const uint32_t n_slots = H * 32;
+  uint64_t total = 0, last_gain = 0;
+  for (uint32_t h = 0; h < H; ++h) total += hcost[h];
   for (uint64_t it = 0; it < iters && H > 1; ++it) {
-    const uint32_t a = (uint32_t)(next() % n_slots), b = (uint32_t)(next() % n_slots);
+    if ((it & 1023u) == 0 && (above_floor() == 0 || it - last_gain > 20000)) break;
+    // first slot: a half-wave above the floor (a few draws), a row of it on a pair loaded
+    // above two in some table; else any slot
+    uint32_t a = (uint32_t)(next() % n_slots);
+    {
+      uint32_t h = (uint32_t)(next() % H), tries = 0;
+      while (hcost[h] <= floor_cost && ++tries < 8) h = (uint32_t)(next() % H);
+      if (hcost[h] > floor_cost) {
+        const uint32_t s0 = (uint32_t)(next() % 32);
+        for (uint32_t k = 0; k < 32; ++k) {
+          const uint32_t r = slot[(size_t)h * 32 + (s0 + k) % 32];
+          if (r == UINT32_MAX) continue;
+          bool hot = false;
+          for (uint32_t f = 0; f < F && !hot; ++f) {
+            const uint8_t c = bin(h, f, bank[(size_t)r * F + f]);
+            hot = c > 2 && c == mx[(size_t)h * FF + f];
+          }
+          if (hot) {
+            a = h * 32 + (s0 + k) % 32;
+            break;
+          }
+        }
+      }
+    }
+    const uint32_t b = (uint32_t)(next() % n_slots);
     const uint32_t ha = a / 32, hb = b / 32;
     const uint32_t ra = slot[a], rb = slot[b];
     if (ha == hb || (ra == UINT32_MAX && rb == UINT32_MAX)) continue;
-    const uint32_t before = cost(ha) + cost(hb);
-    auto move = [&](uint32_t r, uint32_t from, uint32_t to) {
-      if (r == UINT32_MAX) return;
-      for (uint32_t f = 0; f < F; ++f) {
-        --bin(from, f, bank[(size_t)r * F + f]);
-        ++bin(to, f, bank[(size_t)r * F + f]);
-      }
-    };
+    const uint32_t before = hcost[ha] + hcost[hb];
     move(ra, ha, hb);
     move(rb, hb, ha);
-    if (cost(ha) + cost(hb) <= before) {
+    const uint32_t after = hcost[ha] + hcost[hb];
+    if (after <= before) {
       slot[a] = rb;
       slot[b] = ra;
+      if (after < before) {
+        total -= before - after;
+        last_gain = it;
+      }
     } else {
       move(ra, hb, ha);
       move(rb, ha, hb);

@@ -56,6 +56,8 @@ def main() -> int:
                     help="sieve: candidate wheels ranked per k for the time to first block")
     ap.add_argument("--part2-shortlist", type=int, default=8,
                     help="sieve: wheels probed per k (besides the plain one) for it")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="sieve: leave out the short batch (config 3) and filter-pass (config 4) lines")
     ap.add_argument("--no-confirmation", action="store_true",
                     help="sieve: leave out the k=399 confirmation wheel's rate")
     ap.add_argument("--no-part2", action="store_true",
@@ -168,6 +170,7 @@ def main() -> int:
     roofline = sieve_roofline(eng, plan, kern_rate, dev_ms, launches, lds_peak)
     cpu = None if a.no_cpu else cpu_baseline(plan, eng, g_idx, a)
     conf = None if a.no_confirmation else confirmation_wheel(a, lds_peak)
+    configs = None if a.no_configs else other_configs(a)
     part2 = None if a.no_part2 else first_block_ab(candidates=a.part2_candidates,
                                                    shortlist=a.part2_shortlist)
 
@@ -182,9 +185,35 @@ def main() -> int:
                    "strip": eng.strip_len, "threads": eng.threads, "layout": eng.layout,
                    "steps_per_unit": eng.info.steps if eng.layout == "rows" else None},
         "roofline": roofline, "cpu_baseline": cpu, "confirmation_wheel": conf,
-        "time_to_first_block": part2,
+        "time_to_first_block": part2, "other_configs": configs,
     }), flush=True)
     return 0
+
+
+def other_configs(a) -> dict:
+    """Configs 3 and 4 at their full sizes, in short form, so one default run records them:
+    the k=300..320 batch (84 jobs of 2^34 positions, every job's survivors checked against its
+    engine run alone) and the k=397 filter pass over 2^30 candidates (survivors re-checked by
+    the oracle's Kummer test).  `--workload batch` and `--workload fpass` print the full lines."""
+    sub = argparse.Namespace(**vars(a))
+    sub.steps, sub.warmup, sub.no_cpu, sub.autotune, sub.layout = 3, 1, True, False, None
+    b = batch_line(sub)
+    sub.steps = 5
+    f = fpass_line(sub)
+    return {
+        "batch_k300_320": {"value": b["value"], "unit": b["unit"], "ms_per_step": b["ms_per_step"],
+                           "workload": b["config"]["workload"],
+                           "row_lane_plans": b["config"]["row_lane_plans"],
+                           "unbatched_positions_per_s": b["unbatched"]["rate"],
+                           "roofline": {k: b["roofline"][k] for k in
+                                        ("bound", "achieved", "peak", "unit", "frac")}},
+        "filter_pass_k397": {"value": f["value"], "unit": f["unit"],
+                             "workload": f["config"]["workload"],
+                             "survivors": f["config"]["survivors"],
+                             "roofline": {k: f["roofline"][k] for k in
+                                          ("bound", "achieved", "peak", "unit", "frac",
+                                           "kernel_avg_launch_ms")}},
+    }
 
 
 def confirmation_wheel(a, lds_peak: dict, steps: int = 10) -> dict:
@@ -407,6 +436,11 @@ HBM_PEAK = 8.0e12      # bytes/s
 
 
 def bench_fpass(a) -> int:
+    print(json.dumps(fpass_line(a)), flush=True)
+    return 0
+
+
+def fpass_line(a) -> dict:
     """Candidate filter pass: 2^30 u32 IDs into a 2^30-entry u128 stream (k = 397, filter primes
     101..397).  Algorithmic bytes per candidate: 4 (ID) + 16 (value) + 4 x survival (output)."""
     import numpy as np
@@ -446,7 +480,7 @@ def bench_fpass(a) -> int:
         cpu = {"value": m / cdt, "unit": "candidates/s", "cores": 1, "kind": "port",
                "sample": f"first 2^22 stream entries; C Kummer digit test per filter prime; "
                          f"survivors identical to the GPU's ({len(want)})"}
-    print(json.dumps({
+    return {
         "metric": "candidate filter pass throughput (k=397, filter primes 101..397)",
         "value": n / (dev_ms / 1e3), "unit": "candidates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True,
@@ -457,8 +491,7 @@ def bench_fpass(a) -> int:
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_candidate": bytes_per, "kernel_avg_launch_ms": dev_ms},
-        "cpu_baseline": cpu}), flush=True)
-    return 0
+        "cpu_baseline": cpu}
 
 
 def bench_valu(a) -> int:
@@ -532,6 +565,11 @@ def bench_stream(a) -> int:
 
 
 def bench_batch(a) -> int:
+    print(json.dumps(batch_line(a)), flush=True)
+    return 0
+
+
+def batch_line(a) -> dict:
     """k = 300..320 (default planner wheels), 4 seeded blocks per k, windows of 2^34 positions
     at seeded offsets: one batched run (one grid-mapped launch per kernel shape) against the same
     jobs run one engine launch sequence at a time.  Survivors must agree job by job."""
@@ -609,7 +647,7 @@ def bench_batch(a) -> int:
                        "positions_per_s_wall": len(blocks) * len(sw_jobs) * (1 << 28) / wall,
                        "survivors": sum(map(sum, res))}
     assert sweep["eager"]["survivors"] == sweep["graph"]["survivors"]
-    print(json.dumps({
+    return {
         "metric": "batched residue positions sieved/s (k=300..320, 4 blocks each)",
         "value": pos / (bms / 1e3), "unit": "positions/s", "n_gpus": 1, "steps": a.steps,
         "warmup": 1, "ms_per_step": bms, "higher_is_better": True, "scaling": "weak",
@@ -622,8 +660,7 @@ def bench_batch(a) -> int:
         "unbatched": {"device_ms": sms, "wall_s": swall, "rate": pos / (sms / 1e3)},
         "block_sweep": {"jobs": len(sw_jobs), "blocks": len(blocks), "window": 1 << 28,
                         **sweep},
-        "survivors_total": sum(len(x) for x in single)}), flush=True)
-    return 0
+        "survivors_total": sum(len(x) for x in single)}
 
 
 def batch_roofline(engines, jobs, N: int, rate: float, ms: float, launches: int) -> dict:

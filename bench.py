@@ -146,17 +146,16 @@ def main() -> int:
     assert base >= 0
     windows = [(base + i * N, base + (i + 1) * N) for i in range(nsteps)]
 
-    for i in range(a.warmup):
-        eng.scan_window(BLOCK, *windows[i])
+    # each step: one window, its survivors and minimum read back; the engine queues window
+    # i + 1 before reading window i (scan_windows), so the device runs the steps back to back
+    eng.scan_windows(BLOCK, windows[:max(a.warmup, 1)])   # also sets up the result slots
     eng.sync()
     eng.kernel_time()                                   # reset the device timers
-    found = []
     t0 = time.perf_counter()
-    for i in range(a.warmup, nsteps):
-        ids, mn = eng.scan_window(BLOCK, *windows[i])
-        found.append((len(ids), mn))
+    res = eng.scan_windows(BLOCK, windows[a.warmup:])
     eng.sync()
     t1 = time.perf_counter()
+    found = [(len(ids), mn) for ids, mn in res]
     dev_ms, launches, dev_pos = eng.kernel_time()
     wall = t1 - t0
     positions = a.steps * N
@@ -231,12 +230,11 @@ def confirmation_wheel(a, lds_peak: dict, steps: int = 10) -> dict:
     nsteps = 2 + steps
     base = g_idx - (nsteps - 1) * N - N // 2
     windows = [(base + i * N, base + (i + 1) * N) for i in range(nsteps)]
-    for w in windows[:2]:
-        eng.scan_window(BLOCK_CONF, *w)
+    eng.scan_windows(BLOCK_CONF, windows[:2])
     eng.sync()
     eng.kernel_time()
     t0 = time.perf_counter()
-    found = [eng.scan_window(BLOCK_CONF, *w) for w in windows[2:]]
+    found = eng.scan_windows(BLOCK_CONF, windows[2:])
     eng.sync()
     wall = time.perf_counter() - t0
     dev_ms, launches, dev_pos = eng.kernel_time()
@@ -702,15 +700,15 @@ def bench_k200(a) -> int:
     rng = random.Random(200)
     t = rng.randrange(1 << 20)
     lo = rng.randrange(plan.R - N)
-    for _ in range(max(1, a.warmup)):
-        eng.scan_window(t, lo, lo + N)
+    eng.scan_windows(t, [(lo, lo + N)] * max(1, a.warmup))
     eng.sync()
     eng.kernel_time()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ids, mn = eng.scan_window(t, lo, lo + N)
+    res = eng.scan_windows(t, [(lo, lo + N)] * a.steps)
     eng.sync()
     wall = time.perf_counter() - t0
+    assert all(r == res[0] for r in res), "every pass over the window must agree"
+    ids, mn = res[-1]
     dev_ms, launches, dev_pos = eng.kernel_time()
     ns = sorted(plan.idx_to_n(int(i), t) for i in ids)
     cpu = None

@@ -290,6 +290,21 @@ struct es_engine {
   size_t ev_used = 0;
   uint64_t launches = 0, positions = 0;
   uint64_t floor = 0;        // host copy of d_count[2]
+  // es_engine_scan_windows: two result slots (set up on first use), so that window i + 1 runs
+  // while the host reads window i
+  struct Slot {
+    unsigned long long* d_out = nullptr;  // [out_cap] IDs, then [out_cap][2] residues
+    unsigned long long* d_ctr = nullptr;  // count, 1 + slot of the minimum, its floor
+    ColdPlan* d_cold = nullptr;           // the exact test's record with this slot's buffers
+    RowPlan* d_rp = nullptr;              // the row-lane plan pointing at d_cold
+    hipEvent_t done = nullptr;            // the slot's header is written
+  };
+  Slot slot[2];
+  void* slot_mem = nullptr;
+  uint64_t* h_hdr = nullptr;              // pinned, mapped [2][4] headers (count, least s lo,
+                                          // hi, 1 if there is a minimum), then [4] counter set-up
+  uint64_t* h_hdr_dev = nullptr;          // h_hdr as the device addresses it
+  hipStream_t copy_stream = nullptr;      // survivor read-back beside the sieve stream
 };
 
 namespace {
@@ -354,8 +369,11 @@ int launch_kernel(es_engine* e, hipStream_t st, const ColdPlan* cold, const RowP
   return ES_OK;
 }
 
-int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
-  e->count_valid = false;
+// One timed launch on the engine's stream; `cold` / `rows_plan` as in launch_kernel (a result
+// slot of es_engine_scan_windows), or null for the engine's own survivor buffer.
+int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi,
+                 const ColdPlan* cold = nullptr, const RowPlan* rows_plan = nullptr) {
+  if (!cold) e->count_valid = false;
   if (o_hi <= o_lo) return ES_OK;
   hipEvent_t a = nullptr, b = nullptr;
   if (e->ev_used + 2 <= e->ev_pool.size()) {
@@ -364,7 +382,7 @@ int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, ui
     e->ev_used += 2;
     ES_HIP(hipEventRecord(a, e->stream));
   }
-  const int rc = launch_kernel(e, e->stream, nullptr, nullptr, o_lo, o_hi, idx_lo, idx_hi);
+  const int rc = launch_kernel(e, e->stream, cold, rows_plan, o_lo, o_hi, idx_lo, idx_hi);
   if (rc) return rc;
   if (b) {
     ES_HIP(hipEventRecord(b, e->stream));
@@ -375,6 +393,82 @@ int launch_range(es_engine* e, uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, ui
     e->positions += hi > lo ? hi - lo : 0;
   }
   e->launches += 1;
+  return ES_OK;
+}
+
+// Positions [idx_lo, idx_hi) of the current block in equal launches of at most launch_outer
+// outer indices (no short tail launch), each a multiple of lead so it covers whole runs of idx.
+int scan_range(es_engine* e, uint64_t idx_lo, uint64_t idx_hi, const ColdPlan* cold,
+               const RowPlan* rows_plan) {
+  if (idx_lo == idx_hi) return ES_OK;
+  uint64_t o_lo, o_hi;
+  outer_range(e, idx_lo, idx_hi, &o_lo, &o_hi);
+  const uint64_t n = (o_hi - o_lo + e->launch_outer - 1) / e->launch_outer;
+  uint64_t per = (o_hi - o_lo + n - 1) / n;
+  per = (per + e->lead - 1) / e->lead * e->lead;
+  for (uint64_t a = o_lo; a < o_hi; a += per) {
+    const uint64_t b = std::min(o_hi, a + per);
+    const int rc = launch_range(e, a, b, idx_lo, idx_hi, cold, rows_plan);
+    if (rc) return rc;
+  }
+  return ES_OK;
+}
+
+// A result slot's header after its window, written straight to pinned host memory:
+// {count (may exceed the slot's capacity), least residue s lo, hi, 1 if there is a minimum};
+// the counter and the minimum's slot then start again at zero for the slot's next window (its
+// floor stays).
+__global__ void es_slot_finish_kernel(unsigned long long* __restrict__ ctr,
+                                      const unsigned long long* __restrict__ out_s,
+                                      uint64_t* __restrict__ hdr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint64_t best = ctr[1];
+    hdr[0] = ctr[0];
+    hdr[1] = best ? out_s[2 * (best - 1)] : 0;
+    hdr[2] = best ? out_s[2 * (best - 1) + 1] : 0;
+    hdr[3] = best ? 1 : 0;
+    __threadfence_system();
+    ctr[0] = 0;
+    ctr[1] = 0;
+  }
+}
+
+// Both result slots of es_engine_scan_windows: device buffers like the engine's own (IDs and
+// residue records for out_cap survivors, counters), the exact test's record and the row-lane
+// plan pointing at them, the pinned host headers (mapped for the device) and the read-back
+// stream.
+int setup_slots(es_engine* e) {
+  if (e->slot_mem) return ES_OK;
+  const uint64_t cap = std::max<uint64_t>(e->out_cap, 1);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_ctr = al(cap * 24), o_cold = o_ctr + 256, o_rp = o_cold + al(sizeof(ColdPlan)),
+               per = o_rp + al(sizeof(RowPlan));
+  void* mem = nullptr;
+  ES_HIP(hipMalloc(&mem, 2 * per));
+  e->slot_mem = mem;
+  ES_HIP(hipHostMalloc((void**)&e->h_hdr, 16 * sizeof(uint64_t),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  ES_HIP(hipHostGetDevicePointer((void**)&e->h_hdr_dev, e->h_hdr, 0));
+  if (!e->h_hdr_dev) return fail(ES_ERR_DEVICE, "pinned header has no device address");
+  ES_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+  for (int i = 0; i < 2; ++i) {
+    es_engine::Slot& sl = e->slot[i];
+    uint8_t* b = (uint8_t*)mem + i * per;
+    sl.d_out = (unsigned long long*)b;
+    sl.d_ctr = (unsigned long long*)(b + o_ctr);
+    sl.d_cold = (ColdPlan*)(b + o_cold);
+    sl.d_rp = (RowPlan*)(b + o_rp);
+    ES_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    ColdPlan c = e->cold;
+    c.out = sl.d_out;
+    c.out_n = sl.d_ctr;
+    c.out_s = sl.d_out + cap;
+    c.out_cap = e->out_cap;
+    ES_HIP(hipMemcpy(sl.d_cold, &c, sizeof(c), hipMemcpyHostToDevice));
+    RowPlan q = e->rp;
+    q.cold = sl.d_cold;
+    ES_HIP(hipMemcpy(sl.d_rp, &q, sizeof(q), hipMemcpyHostToDevice));
+  }
   return ES_OK;
 }
 
@@ -999,6 +1093,11 @@ int es_engine_destroy(es_engine* e) {
   if (!e) return ES_OK;
   if (e->stream) hipStreamSynchronize(e->stream);
   for (auto& ev : e->ev_pool) hipEventDestroy(ev);
+  for (auto& sl : e->slot)
+    if (sl.done) hipEventDestroy(sl.done);
+  if (e->slot_mem) hipFree(e->slot_mem);
+  if (e->h_hdr) hipHostFree(e->h_hdr);
+  if (e->copy_stream) hipStreamDestroy(e->copy_stream);
   if (e->d_out) hipFree(e->d_out);
   if (e->dblob) hipFree(e->dblob);
   if (e->stream) hipStreamDestroy(e->stream);
@@ -1060,20 +1159,79 @@ int es_engine_reset(es_engine* e) {
 int es_engine_scan_window(es_engine* e, uint64_t idx_lo, uint64_t idx_hi) {
   if (!e) return fail(ES_ERR_INVALID, "null engine");
   if (idx_hi > (uint64_t)e->R || idx_lo > idx_hi) return fail(ES_ERR_INVALID, "window outside [0, R)");
-  if (idx_lo == idx_hi) return ES_OK;
-  uint64_t o_lo, o_hi;
-  outer_range(e, idx_lo, idx_hi, &o_lo, &o_hi);
-  // equal launches of at most launch_outer outer indices (no short tail launch), each a
-  // multiple of lead so it covers whole runs of idx
-  const uint64_t n = (o_hi - o_lo + e->launch_outer - 1) / e->launch_outer;
-  uint64_t per = (o_hi - o_lo + n - 1) / n;
-  per = (per + e->lead - 1) / e->lead * e->lead;
-  for (uint64_t a = o_lo; a < o_hi; a += per) {
-    const uint64_t b = std::min(o_hi, a + per);
-    const int rc = launch_range(e, a, b, idx_lo, idx_hi);
-    if (rc) return rc;
+  return scan_range(e, idx_lo, idx_hi, nullptr, nullptr);
+}
+
+int es_engine_scan_windows(es_engine* e, const uint64_t* bounds, uint32_t n_windows,
+                           uint64_t* ids, uint64_t cap, uint64_t* counts, uint64_t* min_s,
+                           uint8_t* has_min) {
+  if (!e || (n_windows && (!bounds || !counts)) || (cap && !ids))
+    return fail(ES_ERR_INVALID, "null argument");
+  if ((min_s || has_min) && !e->t_known)
+    return fail(ES_ERR_INVALID, "the minimum needs the block index: set it with "
+                                "es_engine_set_block (es_engine_set_tconst leaves it unknown)");
+  for (uint32_t i = 0; i < n_windows; ++i)
+    if (bounds[2 * i + 1] > (uint64_t)e->R || bounds[2 * i] > bounds[2 * i + 1])
+      return fail(ES_ERR_INVALID, "window outside [0, R)");
+  if (n_windows == 0) return ES_OK;
+  int rc = setup_slots(e);
+  if (rc) return rc;
+  // both slots start from zero with the block's floor (the least s that counts for the minimum)
+  uint64_t* init = e->h_hdr + 8;
+  init[0] = 0;
+  init[1] = 0;
+  init[2] = e->floor;
+  init[3] = 0;
+  for (auto& sl : e->slot)
+    ES_HIP(hipMemcpyAsync(sl.d_ctr, init, 32, hipMemcpyHostToDevice, e->stream));
+  ES_HIP(hipStreamSynchronize(e->stream));
+  auto submit = [&](uint32_t i) -> int {
+    es_engine::Slot& sl = e->slot[i & 1];
+    const int r = scan_range(e, bounds[2 * i], bounds[2 * i + 1], sl.d_cold, sl.d_rp);
+    if (r) return r;
+    hipLaunchKernelGGL(es_slot_finish_kernel, dim3(1), dim3(64), 0, e->stream, sl.d_ctr,
+                       (const unsigned long long*)(sl.d_out + std::max<uint64_t>(e->out_cap, 1)),
+                       e->h_hdr_dev + 4 * (i & 1));
+    ES_HIP(hipGetLastError());
+    ES_HIP(hipEventRecord(sl.done, e->stream));
+    return ES_OK;
+  };
+  // window i's results, read while window i + 1 runs; errors are kept until the stream drains
+  uint64_t used = 0;
+  int err = ES_OK;
+  auto finish = [&](uint32_t i) -> int {
+    es_engine::Slot& sl = e->slot[i & 1];
+    ES_HIP(hipEventSynchronize(sl.done));
+    const uint64_t* h = e->h_hdr + 4 * (i & 1);
+    const uint64_t c = h[0];
+    counts[i] = c;
+    if (min_s) {
+      min_s[2 * i] = h[1];
+      min_s[2 * i + 1] = h[2];
+    }
+    if (has_min) has_min[i] = (uint8_t)h[3];
+    if (c > e->out_cap || c > cap - std::min(cap, used)) {
+      if (!err) err = fail(ES_ERR_OVERFLOW, "survivor buffer overflow");
+      return ES_OK;
+    }
+    if (c && !err) {
+      ES_HIP(hipMemcpyAsync(ids + used, sl.d_out, c * 8, hipMemcpyDeviceToHost, e->copy_stream));
+      ES_HIP(hipStreamSynchronize(e->copy_stream));
+    }
+    used += c;
+    return ES_OK;
+  };
+  rc = submit(0);
+  for (uint32_t i = 1; i < n_windows && !rc; ++i) {
+    rc = submit(i);
+    if (!rc) rc = finish(i - 1);
   }
-  return ES_OK;
+  if (rc) {  // a launch failed: let the windows already queued finish before returning
+    hipStreamSynchronize(e->stream);
+    return rc;
+  }
+  rc = finish(n_windows - 1);
+  return rc ? rc : err;
 }
 
 int es_engine_scan(es_engine* e, uint64_t ostart, uint64_t ocount) {

@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
         "es_engine_reset": (C.c_int, [eng]),
         "es_engine_scan": (C.c_int, [eng, C.c_uint64, C.c_uint64]),
         "es_engine_scan_window": (C.c_int, [eng, C.c_uint64, C.c_uint64]),
+        "es_engine_scan_windows": (C.c_int, [eng, u64p, C.c_uint32, u64p, C.c_uint64, u64p,
+                                             u64p, C.POINTER(C.c_uint8)]),
         "es_engine_count": (C.c_int, [eng, u64p]),
         "es_engine_read": (C.c_int, [eng, C.c_uint64, C.c_uint64, u64p]),
         "es_engine_collect": (C.c_int, [eng, u64p, C.c_uint64, u64p, u64p, C.POINTER(C.c_int)]),

@@ -731,6 +731,36 @@ class GpuSieve:
             return None
         return self.collect(t)
 
+    def scan_windows(self, t: int, windows, cap: int | None = None):
+        """(ids, min_n) of each window (idx_lo, idx_hi) of block t, in order, as scan_window
+        returns them one by one; the library queues each window before reading the previous
+        one back, so the device runs them back to back.  cap bounds the IDs of all windows
+        together (default: out_cap)."""
+        ws = [(int(a), int(b)) for a, b in windows]
+        if not ws:
+            return []
+        self.set_block(t)
+        cap = self.out_cap if cap is None else int(cap)
+        bounds = np.asarray(ws, dtype=np.uint64).reshape(-1)
+        ids = np.zeros(max(cap, 1), dtype=np.uint64)
+        counts = np.zeros(len(ws), dtype=np.uint64)
+        ms = np.zeros(2 * len(ws), dtype=np.uint64)
+        has = np.zeros(len(ws), dtype=np.uint8)
+        rc = nat.lib().es_engine_scan_windows(
+            self._h, nat.ptr64(bounds), len(ws), nat.ptr64(ids), cap, nat.ptr64(counts),
+            nat.ptr64(ms), has.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc == nat.ES_ERR_OVERFLOW:
+            raise RuntimeError(f"survivor buffer overflow (window counts {counts.tolist()}, "
+                               f"cap {cap}, out_cap {self.out_cap}); raise max_survivors")
+        nat.check(rc)
+        self.residues_checked += sum(b - a for a, b in ws)
+        out, at = [], 0
+        for i, c in enumerate(counts.tolist()):
+            mn = t * self.plan.M + (int(ms[2 * i + 1]) << 64 | int(ms[2 * i])) if has[i] else None
+            out.append(([int(x) for x in ids[at:at + c]], mn))
+            at += c
+        return out
+
     def collect(self, t: int):
         n = C.c_uint64(0)
         nat.check(nat.lib().es_engine_count(self._h, C.byref(n)))

@@ -144,6 +144,21 @@ int es_engine_reset(es_engine* e);
 int es_engine_scan(es_engine* e, uint64_t ostart, uint64_t ocount);
 /* Scan positions [idx_lo, idx_hi) of the current block.  Asynchronous. */
 int es_engine_scan_window(es_engine* e, uint64_t idx_lo, uint64_t idx_hi);
+/* Scan windows bounds[2i] <= idx < bounds[2i+1] (i < n_windows) of the current block one after
+ * another, each with its own survivors and minimum, as es_engine_reset + es_engine_scan_window
+ * + es_engine_collect would per window; synchronous.  Window i + 1 is queued before window i's
+ * results are read back (two result slots of out_cap survivors each, set up on the first call),
+ * so the device runs the windows back to back.  The survivor IDs of all windows go to ids in
+ * window order (at most cap in total), counts[i] = survivors of window i, min_s[2i..2i+1] =
+ * least residue s (n = tM + s > k+1) of window i and has_min[i] = 1 if it has one (min_s and
+ * has_min may be NULL; with either given the block index must be known).  ES_ERR_OVERFLOW when
+ * a window exceeds out_cap or the IDs exceed cap; counts stay valid.  The engine's own survivor
+ * buffer (es_engine_count / es_engine_collect) is left as it was.  No reference counterpart:
+ * the reference reads back after every block (sieve/gpu_sieve_v9.py:545-568); this is for
+ * callers that walk many windows of one block. */
+int es_engine_scan_windows(es_engine* e, const uint64_t* bounds, uint32_t n_windows,
+                           uint64_t* ids, uint64_t cap, uint64_t* counts, uint64_t* min_s,
+                           uint8_t* has_min);
 /* Synchronise; *n = survivors appended since the last reset (may exceed out_cap). */
 int es_engine_count(es_engine* e, uint64_t* n);
 /* Synchronise; copy survivor IDs [offset, offset+count) (count <= out_cap - offset). */

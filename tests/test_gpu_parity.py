@@ -517,3 +517,63 @@ def test_row_lane_wider_rows(k, wheel):
         with pytest.raises(Exception, match="two words"):
             G.GpuSieve(EP.Plan(400, {"mi_inner_radix": 128, "mi_ring_swap": 0,
                                      "mi_layout": "rows"}))
+
+
+def _windows_against_oracle(eng, plan, k, filt, t, windows):
+    got = eng.scan_windows(t, windows)
+    assert len(got) == len(windows)
+    total = 0
+    for (lo, hi), (ids, mn) in zip(windows, got):
+        n, want, wmin = O.c_scan_min(k, plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 22)
+        assert len(ids) == n, f"window [{lo},{hi}): count {len(ids)} != oracle {n}"
+        assert np.array_equal(np.sort(np.asarray(ids, dtype=np.uint64)), want)
+        assert mn == wmin, f"window [{lo},{hi}): minimum {mn} != oracle {wmin}"
+        total += n
+    return total
+
+
+@pytest.mark.parametrize("k,wheel,tune", [
+    (30, C1_WHEEL, {}),                                             # strip-lane, dense
+    (399, K399_PRIMARY, {"mi_filter_limit": 4}),                    # row-lane, one word
+    (309, None, {"mi_filter_limit": 8, "mi_ring_swap": 1}),         # lead block
+    (399, K399_CONFIRM, {"mi_filter_limit": 4, "mi_ring_swap": 0,   # two-word rows
+                         "mi_inner_radix": 128, "mi_layout": "rows"}),
+])
+def test_scan_windows_pipelined(k, wheel, tune):
+    """es_engine_scan_windows (each window queued before the previous one is read back, two
+    result slots) returns per window exactly the oracle's survivors and minimum: dense,
+    empty, unaligned and one-position windows, an odd number of windows (both slots reused),
+    block 0 (s <= k+1 left out of the minimum) and a later block."""
+    eng, plan = engine(k, wheel, max_survivors=1 << 20, **tune)
+    filt = plan.rest[:tune["mi_filter_limit"]] if "mi_filter_limit" in tune else plan.rest
+    rng = random.Random(k)
+    base = plan.R // 7
+    span = 1 << 18 if k == 30 else 1 << 20
+    windows = [(base, base + span), (base + span, base + span), (base + span, base + 2 * span + 17),
+               (base + 5 * span + 3, base + 5 * span + 4)]
+    windows += [(lo, lo + rng.randrange(1, span)) for lo in
+                (rng.randrange(0, plan.R - span) for _ in range(3))]
+    for t in (0, 6):
+        say(f"case scan_windows k={k} t={t} layout={eng.layout} windows={len(windows)} "
+            f"r0={eng.r0} W={eng.info.words}")
+        assert _windows_against_oracle(eng, plan, k, filt, t, windows) > 0
+    # the first positions of block 0 (the floor) and the engine's own buffer left untouched
+    eng.scan_window(0, 0, span, collect=False)
+    mine = eng.collect(0)
+    _windows_against_oracle(eng, plan, k, filt, 0, [(0, span), (span, 2 * span)])
+    assert eng.collect(0) == mine
+
+
+def test_scan_windows_overflow_and_errors():
+    eng, plan = engine(30, C1_WHEEL, max_survivors=1 << 16)
+    say("case scan_windows overflow k=30")
+    lo = 1 << 22
+    with pytest.raises(RuntimeError, match="overflow"):       # one window above out_cap
+        eng.scan_windows(0, [(0, 1 << 24)])
+    with pytest.raises(RuntimeError, match="overflow"):       # the windows together above cap
+        eng.scan_windows(0, [(lo, lo + (1 << 16)), (lo + (1 << 16), lo + (1 << 17))], cap=100)
+    with pytest.raises(Exception, match="outside"):
+        eng.scan_windows(0, [(0, plan.R + 1)])
+    assert eng.scan_windows(0, []) == []
+    # the engine recovers: the next call is exact
+    _windows_against_oracle(eng, plan, 30, plan.rest, 2, [(lo, lo + (1 << 14))] * 3)

@@ -7,8 +7,8 @@
 // the original ID is what is emitted.
 //
 // Per candidate: one coalesced 4-byte ID load and one 16-byte value load, then
-// n mod q from the value's four 32-bit limbs in FP64 (sum of limb * (2^32k mod q) < 2^52, so
-// it is exact; quotient by one multiply with 1/q and one fused multiply-subtract, corrected
+// n mod q from the value's four 32-bit limbs in FP64 (sum of limb * (2^32k mod q) < 2^52 for
+// q < 2^18, so it is exact; quotient by one multiply with 1/q and one fused multiply-subtract, corrected
 // by at most one q), and a bit test in the filter's table.  Tables of the most selective
 // filters sit in LDS.  Candidates surviving the first filters are packed per wave before the
 // rest are tested (es_filter_kernel).  Survivors are compacted with a wave ballot: one atomic
@@ -27,7 +27,9 @@ namespace {
 
 // Filter record, 6 doubles: 2^32, 2^64, 2^96 mod q, q, 1/q, then (u32 table word offset, u32
 // in-LDS flag).  (Testing a Kummer table by the base-p digits of n mod q instead of reading it
-// was measured slower: the pass's first filters are bound by their FP64 arithmetic.)
+// was measured slower.  So was a residue left in [0, 3q) without the two corrections, read
+// from tables holding each bitmap three times: 4.30 ms against 4.16-4.23 at 2^30 candidates,
+// the larger tables costing more than the arithmetic saved.)
 constexpr int kFiltRecDw = 12;
 
 __device__ __forceinline__ uint32_t fp_mod(double d0, double d1, double d2, double d3,
@@ -269,7 +271,8 @@ int es_fpass_create(uint32_t n_filters, const uint32_t* filt_q, const uint32_t* 
   uint32_t nl = 0;
   for (uint32_t j = 0; j < n_filters; ++j) {
     const uint32_t q = filt_q[j];
-    if (q < 2 || q >= (1u << 24)) return fail(ES_ERR_LIMIT, "filter modulus outside [2, 2^24)");
+    // q < 2^18 keeps the FP64 limb sum below 2^52 (exact)
+    if (q < 2 || q >= (1u << 18)) return fail(ES_ERR_LIMIT, "filter modulus outside [2, 2^18)");
     const uint32_t nw = (q + 63) / 64;
     if ((uint64_t)filt_off[j] + nw > n_words) return fail(ES_ERR_INVALID, "bitmap outside words");
     const bool to_lds = (j == nl) && lds_img.size() + nw <= budget;

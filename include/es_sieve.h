@@ -229,7 +229,8 @@ int es_batch_replay(es_batch* b, double* ms);
  * every filter table (bit n mod q_j set) is emitted, in unspecified order. */
 typedef struct es_fpass es_fpass;
 
-/* Filters in test order (most selective first); tables are u64 words, bit r = residue r. */
+/* Filters in test order (most selective first); tables are u64 words, bit r = residue r;
+ * moduli q in [2, 2^18) (ES_ERR_LIMIT otherwise: the residue arithmetic is exact below). */
 int es_fpass_create(uint32_t n_filters, const uint32_t* filt_q, const uint32_t* filt_off,
                     const uint64_t* words, uint32_t n_words, uint32_t lds_budget,
                     uint64_t out_cap, es_fpass** out);

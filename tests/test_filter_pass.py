@@ -205,6 +205,11 @@ def test_gpu_pass_arbitrary_bitmaps():
     vals = _spec_values(spec, 1 << 14, 13)
     # values that hit the changed residues of filter 3, so both edits matter
     vals += [on + q3 * i for i in range(1, 200)] + [off + q3 * i for i in range(1, 200)]
+    # edges of the u128 range and of every modulus (the residue's exactness and its range)
+    top = (1 << 128) - 1
+    vals += [0, 1, top, top - 1, 1 << 127, (1 << 96) - 1, 1 << 64, (1 << 32) - 1]
+    for q in qs:
+        vals += [q - 1, q, q + 1, top - top % q, top - top % q - 1, (1 << 100) // q * q]
     ids = np.random.default_rng(14).permutation(len(vals)).astype(np.uint32)
     print(f"case fpass forms n={len(vals)} q3={q3}", flush=True)
     fp.upload(vals, ids)

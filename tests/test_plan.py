@@ -40,6 +40,22 @@ def test_default_plans_k270_275_280():
             assert s[key] == want[key], (k, key)
 
 
+def test_default_plans_every_k():
+    """The default planner for every k = 2..400 equals the reference planner's: wheel (ring
+    order), M, R, radices, filter order and the fp64 summary fields bit for bit.  The vectors
+    were written by tests/golden/make_plan_vectors.py, which imports the reference's planner
+    module in the build container (recorded plans pin only 39 of these k)."""
+    rec = json.load(open(os.path.join(GOLDEN, "plan_vectors.json")))["plans"]
+    assert [r["k"] for r in rec] == list(range(2, 401))
+    for r in rec:
+        plan = EP.Plan(r["k"])
+        s = json.loads(json.dumps(plan.summary()))
+        for key in ("wheel", "M", "n_filters", "residues_per_block", "density_wheel",
+                    "density_rest", "E_g", "expected_checks_true"):
+            assert s[key] == r[key], (r["k"], key)
+        assert list(plan.radix) == r["radix"] and list(plan.rest) == r["rest"], r["k"]
+
+
 def test_recorded_frontier_plans():
     """Every recorded primary plan (default planner) and every banned-prime confirmation plan
     (harness/run_k_masks.py:250-258) is reproduced exactly, float summary fields included."""

@@ -490,14 +490,15 @@ class _TConst:
 
 def timed_rate(scan, R: int, strip_positions: int, max_s: float = 0.05,
                strips_log2: int = 19, lo_log2: int = 28, hi_log2: int = 34,
-               floor: float = 0.0) -> float:
+               floor: float = 0.0, with_window: bool = False):
     """Positions per second of a layout, from scan(lo, hi) -> rate (device time) on windows
     centred in [0, R).  The target window holds 2^strips_log2 strips (about 8 workgroups per CU,
     so long strips are not timed on an under-filled GPU), clamped to [2^lo_log2, 2^hi_log2]
     positions.  Windows grow from 2^24 positions, each to what the latest rate says fits in
     max_s, until the target is reached or the next window would not grow by a quarter; the
-    last rate is returned.  A layout measuring below `floor` (half the best rate so far, say)
-    after its second window stops there: it cannot win."""
+    last rate is returned (with its window (lo, hi) when with_window).  A layout measuring
+    below `floor` (half the best rate so far, say) after its second window stops there: it
+    cannot win."""
     n = max(1, min(R // 2, 1 << hi_log2, max(1 << lo_log2, strip_positions << strips_log2)))
     w = max(1, min(n, 1 << 24))
     rate = scan((R - w) // 2, (R - w) // 2 + w)
@@ -509,7 +510,7 @@ def timed_rate(scan, R: int, strip_positions: int, max_s: float = 0.05,
             break
         w = min(n, want)
         rate = scan((R - w) // 2, (R - w) // 2 + w)
-    return rate
+    return (rate, ((R - w) // 2, (R - w) // 2 + w)) if with_window else rate
 
 
 class GpuSieve:
@@ -604,48 +605,64 @@ class GpuSieve:
         return opts
 
     @staticmethod
-    def _choose_layout(P: Plan, strips_log2: int = 19) -> tuple[int, object, str]:
+    def _choose_layout(P: Plan, strips_log2: int = 19, rounds: int = 3) -> tuple[int, object, str]:
         """(mi_inner_radix, mi_ring_swap, mi_layout) for the engine: with one layout option,
         that one; otherwise each option is timed on a window of block 1 and the fastest kept.
         Every option scans the same positions with the same exact test; only the LDS layout
         (words per row, masks that fit, strip-lane or row-lane lanes) and so the speed differ.
         An option the library refuses (a row-lane layout without the strip or steps it needs)
-        is skipped.  Each
-        window holds 2^strips_log2 strips of its own layout (about 8 workgroups per CU, so long
-        strips are not timed on an under-filled GPU), between 2^28 and 2^34 positions."""
+        is skipped.  Each option's window grows to 2^strips_log2 strips of its own layout
+        (about 8 workgroups per CU, so long strips are not timed on an under-filled GPU),
+        between 2^28 and 2^34 positions; the options within 15 % of the fastest are then timed
+        again on their windows in `rounds` round-robin passes and the best rate of each counts,
+        so that neither the order of the options (the first ones run on a colder GPU) nor one
+        launch's noise decides: single launches vary by several per cent."""
         opts = GpuSieve._layout_options(P)
         if len(opts) == 1:
             return opts[0]
         L = nat.lib()
-        best, best_rate = opts[0], 0.0
-        for bound, swap, lay in opts:
-            try:
-                d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap, mi_layout=lay))
-            except AssertionError:
-                continue
-            h = C.c_void_p()
-            if L.es_engine_create(C.byref(d.desc), C.byref(h)) != nat.ES_OK:
-                continue
-            try:
+
+        def scanner(h):
+            def scan(lo, hi):
+                ms = C.c_double(0)
+                la, pos = C.c_uint64(0), C.c_uint64(0)
+                nat.check(L.es_engine_set_block(h, 1))
+                nat.check(L.es_engine_reset(h))
+                nat.check(L.es_engine_scan_window(h, lo, hi))
+                nat.check(L.es_engine_sync(h))
+                nat.check(L.es_engine_kernel_time(h, C.byref(ms), C.byref(la), C.byref(pos)))
+                return pos.value / max(ms.value * 1e-3, 1e-12)
+            return scan
+
+        timed = []          # [option, handle, window, best rate]
+        try:
+            best_rate = 0.0
+            for bound, swap, lay in opts:
+                try:
+                    d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap,
+                                      mi_layout=lay))
+                except AssertionError:
+                    continue
+                h = C.c_void_p()
+                if L.es_engine_create(C.byref(d.desc), C.byref(h)) != nat.ES_OK:
+                    continue
+                timed.append([(bound, swap, lay), h, None, 0.0])
                 info = nat.EngineInfo()
                 nat.check(L.es_engine_info_get(h, C.byref(info)))
-
-                def scan(lo, hi):
-                    ms = C.c_double(0)
-                    la, pos = C.c_uint64(0), C.c_uint64(0)
-                    nat.check(L.es_engine_set_block(h, 1))
-                    nat.check(L.es_engine_reset(h))
-                    nat.check(L.es_engine_scan_window(h, lo, hi))
-                    nat.check(L.es_engine_sync(h))
-                    nat.check(L.es_engine_kernel_time(h, C.byref(ms), C.byref(la), C.byref(pos)))
-                    return pos.value / max(ms.value * 1e-3, 1e-12)
-                rate = timed_rate(scan, P.R, info.strip_len * info.r0, max_s=0.05,
-                                  floor=0.5 * best_rate)
-                if rate > best_rate:
-                    best, best_rate = (bound, swap, lay), rate
-            finally:
-                L.es_engine_destroy(h)
-        return best
+                rate, win = timed_rate(scanner(h), P.R, info.strip_len * info.r0, max_s=0.05,
+                                       floor=0.5 * best_rate, with_window=True)
+                timed[-1][2:] = [win, rate]
+                best_rate = max(best_rate, rate)
+            if not timed:
+                return opts[0]
+            close = [t for t in timed if t[3] >= 0.85 * best_rate]
+            for _ in range(rounds if len(close) > 1 else 0):
+                for t in close:
+                    t[3] = max(t[3], scanner(t[1])(*t[2]))
+            return max(close, key=lambda t: t[3])[0]
+        finally:
+            for t in timed:
+                L.es_engine_destroy(t[1])
 
     # -- reference contract ------------------------------------------------------------
     def _launch(self, ostart: int, ocount: int) -> None:

@@ -72,19 +72,19 @@ KernelFn pick_kernel(int F, uint32_t W, uint32_t G, uint32_t C) {
   if (F < kPartFirstF[3]) return pick_kernel_p2(F, W, G, C);
   return pick_kernel_p3(F, W, G, C);
 }
-RowsFn pick_rows(int F, uint32_t C, uint32_t W) {
+RowsFn pick_rows(int F, uint32_t C, uint32_t W, bool wave8) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (F < kPartFirstF[1]) return pick_rows_p0(F, C, W);
-  if (F < kPartFirstF[2]) return pick_rows_p1(F, C, W);
-  if (F < kPartFirstF[3]) return pick_rows_p2(F, C, W);
-  return pick_rows_p3(F, C, W);
+  if (F < kPartFirstF[1]) return pick_rows_p0(F, C, W, wave8);
+  if (F < kPartFirstF[2]) return pick_rows_p1(F, C, W, wave8);
+  if (F < kPartFirstF[3]) return pick_rows_p2(F, C, W, wave8);
+  return pick_rows_p3(F, C, W, wave8);
 }
-RowsBatchFn pick_rows_batch(int F, uint32_t C, uint32_t W) {
+RowsBatchFn pick_rows_batch(int F, uint32_t C, uint32_t W, bool wave8) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (F < kPartFirstF[1]) return pick_rows_batch_p0(F, C, W);
-  if (F < kPartFirstF[2]) return pick_rows_batch_p1(F, C, W);
-  if (F < kPartFirstF[3]) return pick_rows_batch_p2(F, C, W);
-  return pick_rows_batch_p3(F, C, W);
+  if (F < kPartFirstF[1]) return pick_rows_batch_p0(F, C, W, wave8);
+  if (F < kPartFirstF[2]) return pick_rows_batch_p1(F, C, W, wave8);
+  if (F < kPartFirstF[3]) return pick_rows_batch_p2(F, C, W, wave8);
+  return pick_rows_batch_p3(F, C, W, wave8);
 }
 
 // Lane slots of the strip rows for the row-lane layout: G groups of 64 slots, i.e. 2G
@@ -286,6 +286,7 @@ struct es_engine {
   RowPlan rp{};
   RowPlan* d_rp = nullptr;   // device copy of rp (the row-lane kernel takes a pointer)
   RowsFn rows_kern = nullptr;
+  bool wave8 = false;        // row-lane kernel instance sized for 8 waves per SIMD (else 6)
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   uint64_t launches = 0, positions = 0;
@@ -964,34 +965,50 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   hipDeviceProp_t prop;
   ES_HIP(hipGetDeviceProperties(&prop, e->device));
   if (rows) {
-    e->rows_kern = pick_rows((int)e->F, e->kind, W);
+    e->rows_kern = pick_rows((int)e->F, e->kind, W, false);
     if (!e->rows_kern) return fail(ES_ERR_LIMIT, "no row-lane kernel for this mask count");
     e->kern = nullptr;
   } else {
     e->kern = pick_kernel((int)e->F, W, G, e->kind);
     if (!e->kern) return fail(ES_ERR_LIMIT, "no kernel for this mask count");
   }
-  const void* kfn = rows ? (const void*)e->rows_kern : (const void*)e->kern;
-  if (!e->threads) {
-    // The workgroup size that keeps the most waves resident per CU (ties: the smaller one).
-    // Every workgroup holds its own copy of the tables, so large tables favour larger
-    // workgroups; registers and the CU's wave slots bound the rest.
-    uint32_t best_waves = 0;
-    for (uint32_t t : {256u, 512u, 1024u}) {
-      const uint32_t lds_t = table_bytes + (t / 64) * kWaveQueueBytes;
-      if (lds_t > prop.sharedMemPerBlock) continue;
-      if (lds_t > 65536)
-        ES_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)prop.sharedMemPerBlock));
-      int nb = 0;
-      ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, (int)t, lds_t));
-      if ((uint32_t)nb * (t / 64) > best_waves) {
-        best_waves = (uint32_t)nb * (t / 64);
-        e->threads = t;
+  {
+    // The workgroup size (unless given) and, for one-word row-lane rows, the kernel instance
+    // (registers for 6 or 8 waves per SIMD) that keep the most waves resident per CU; ties go
+    // to the 6-wave instance (no spills), then to the smaller workgroup.  Every workgroup
+    // holds its own copy of the tables, so large tables favour larger workgroups; registers
+    // and the CU's wave slots bound the rest.
+    struct Cand { const void* fn; bool wave8; };
+    std::vector<Cand> cands{{rows ? (const void*)e->rows_kern : (const void*)e->kern, false}};
+    if (rows && W == 1) cands.push_back({(const void*)pick_rows((int)e->F, e->kind, W, true), true});
+    const std::vector<uint32_t> sizes = e->threads ? std::vector<uint32_t>{e->threads}
+                                                   : std::vector<uint32_t>{256u, 512u, 1024u};
+    uint32_t best_waves = 0, best_t = 0;
+    bool best_w8 = false;
+    for (const Cand& c : cands) {
+      if (!c.fn) continue;
+      for (uint32_t t : sizes) {
+        const uint32_t lds_t = table_bytes + (t / 64) * kWaveQueueBytes;
+        if (lds_t > prop.sharedMemPerBlock) continue;
+        if (lds_t > 65536)
+          ES_HIP(hipFuncSetAttribute(c.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)prop.sharedMemPerBlock));
+        int nb = 0;
+        ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, c.fn, (int)t, lds_t));
+        if ((uint32_t)nb * (t / 64) > best_waves) {
+          best_waves = (uint32_t)nb * (t / 64);
+          best_t = t;
+          best_w8 = c.wave8;
+        }
       }
     }
-    if (!e->threads) e->threads = 256;  // nothing fits: the check below reports it
+    if (!e->threads) e->threads = best_t ? best_t : 256;  // nothing fits: reported below
+    if (best_w8) {
+      e->wave8 = true;
+      e->rows_kern = pick_rows((int)e->F, e->kind, W, true);
+    }
   }
+  const void* kfn = rows ? (const void*)e->rows_kern : (const void*)e->kern;
   e->lds_bytes = table_bytes + (e->threads / 64) * kWaveQueueBytes;
   if (e->lds_bytes > prop.sharedMemPerBlock)
     return fail(ES_ERR_LIMIT, "LDS tables and candidate queues exceed the device's "
@@ -1537,7 +1554,7 @@ int prepare_batch(es_batch* b, std::vector<BatchLaunch>& out) {
   std::vector<Key> keys;
   for (uint32_t i = 0; i < b->jobs.size(); ++i) {
     const es_engine* e = b->jobs[i].e;
-    const void* fn = e->rows ? (const void*)pick_rows_batch((int)e->F, e->kind, e->W)
+    const void* fn = e->rows ? (const void*)pick_rows_batch((int)e->F, e->kind, e->W, e->wave8)
                              : (const void*)pick_grid_batch((int)e->F, e->W, 0, e->kind);
     if (!fn) return fail(ES_ERR_LIMIT, "no batched kernel for this mask count");
     const uint32_t occ = std::min<uint32_t>(2048u / e->threads,

@@ -781,9 +781,14 @@ __device__ __forceinline__ void es_rows_unit(const RowPlan& P, const uint32_t* l
 }
 
 // Units g + G * (w - w_lo) for w in [w_lo, w_hi), one per wave: the grid covers them all (the
-// host splits a range whose units exceed one grid).
-template <int F, int C, int W>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(W == 1 ? 6 : 5)))
+// host splits a range whose units exceed one grid).  WPE: waves per SIMD the registers are
+// sized for.  One-word rows come at 6 (78 VGPRs, no spills) and at 8 (64 VGPRs; the exact
+// re-test of the drain spills 5-24 of them, the step loop none): the engine takes the 8-wave
+// instance only where it keeps more waves resident, i.e. where two 1024-thread workgroups'
+// tables fit the CU's LDS (k=399: 2.07 against 2.18 ms per 2^36 launch; where the occupancy
+// does not change the spills cost up to 4 %).
+template <int F, int C, int W, int WPE>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE)))
 es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uint64_t o_lo,
                uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
   const RowPlan& P = *Pp;
@@ -803,8 +808,8 @@ es_rows_kernel(const RowPlan* __restrict__ Pp, uint64_t w_lo, uint64_t w_hi, uin
 
 // Grid-mapped batch of row-lane jobs: workgroup b serves units (b - wg_start) * waves + wave
 // of the job whose workgroup range holds b.  GridJob's u_lo/u_hi hold the job's superstrips.
-template <int F, int C, int W>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(W == 1 ? 6 : 5)))
+template <int F, int C, int W, int WPE>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE)))
 es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restrict__ jobs,
                      uint32_t n_jobs) {
   extern __shared__ uint32_t lds[];
@@ -831,30 +836,30 @@ es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restric
 typedef void (*RowsFn)(const RowPlan*, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t);
 typedef void (*RowsBatchFn)(const RowPlan*, const GridJob*, uint32_t);
 
-// Row-lane instances: W = 1 with each carry form and lead-block form (C = 0..3); W = 2 with
-// the byte carry table (C = 1).
+// Row-lane instances: W = 1 with each carry form and lead-block form (C = 0..3), sized for 6
+// or (wave8) 8 waves per SIMD; W = 2 with the byte carry table (C = 1), 5 waves.
 template <int F>
-RowsFn rows_kernel_for(uint32_t C, uint32_t W) {
+RowsFn rows_kernel_for(uint32_t C, uint32_t W, bool wave8) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (W != 1) return W == 2 && C == 1 ? es_rows_kernel<F, 1, 2> : nullptr;
+  if (W != 1) return W == 2 && C == 1 && !wave8 ? es_rows_kernel<F, 1, 2, 5> : nullptr;
   switch (C) {
-    case 0: return es_rows_kernel<F, 0, 1>;
-    case 1: return es_rows_kernel<F, 1, 1>;
-    case 2: return es_rows_kernel<F, 2, 1>;
-    case 3: return es_rows_kernel<F, 3, 1>;
+    case 0: return wave8 ? es_rows_kernel<F, 0, 1, 8> : es_rows_kernel<F, 0, 1, 6>;
+    case 1: return wave8 ? es_rows_kernel<F, 1, 1, 8> : es_rows_kernel<F, 1, 1, 6>;
+    case 2: return wave8 ? es_rows_kernel<F, 2, 1, 8> : es_rows_kernel<F, 2, 1, 6>;
+    case 3: return wave8 ? es_rows_kernel<F, 3, 1, 8> : es_rows_kernel<F, 3, 1, 6>;
     default: return nullptr;
   }
 }
 
 template <int F>
-RowsBatchFn rows_batch_for(uint32_t C, uint32_t W) {
+RowsBatchFn rows_batch_for(uint32_t C, uint32_t W, bool wave8) {
   if (F < 1 || F > kRowsMaxMasks) return nullptr;
-  if (W != 1) return W == 2 && C == 1 ? es_rows_batch_kernel<F, 1, 2> : nullptr;
+  if (W != 1) return W == 2 && C == 1 && !wave8 ? es_rows_batch_kernel<F, 1, 2, 5> : nullptr;
   switch (C) {
-    case 0: return es_rows_batch_kernel<F, 0, 1>;
-    case 1: return es_rows_batch_kernel<F, 1, 1>;
-    case 2: return es_rows_batch_kernel<F, 2, 1>;
-    case 3: return es_rows_batch_kernel<F, 3, 1>;
+    case 0: return wave8 ? es_rows_batch_kernel<F, 0, 1, 8> : es_rows_batch_kernel<F, 0, 1, 6>;
+    case 1: return wave8 ? es_rows_batch_kernel<F, 1, 1, 8> : es_rows_batch_kernel<F, 1, 1, 6>;
+    case 2: return wave8 ? es_rows_batch_kernel<F, 2, 1, 8> : es_rows_batch_kernel<F, 2, 1, 6>;
+    case 3: return wave8 ? es_rows_batch_kernel<F, 3, 1, 8> : es_rows_batch_kernel<F, 3, 1, 6>;
     default: return nullptr;
   }
 }
@@ -911,14 +916,14 @@ GridBatchFn pick_grid_batch_p0(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p1(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p2(int F, uint32_t W, uint32_t G, uint32_t C);
 GridBatchFn pick_grid_batch_p3(int F, uint32_t W, uint32_t G, uint32_t C);
-RowsFn pick_rows_p0(int F, uint32_t C, uint32_t W);
-RowsFn pick_rows_p1(int F, uint32_t C, uint32_t W);
-RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W);
-RowsFn pick_rows_p3(int F, uint32_t C, uint32_t W);
-RowsBatchFn pick_rows_batch_p0(int F, uint32_t C, uint32_t W);
-RowsBatchFn pick_rows_batch_p1(int F, uint32_t C, uint32_t W);
-RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W);
-RowsBatchFn pick_rows_batch_p3(int F, uint32_t C, uint32_t W);
+RowsFn pick_rows_p0(int F, uint32_t C, uint32_t W, bool wave8);
+RowsFn pick_rows_p1(int F, uint32_t C, uint32_t W, bool wave8);
+RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W, bool wave8);
+RowsFn pick_rows_p3(int F, uint32_t C, uint32_t W, bool wave8);
+RowsBatchFn pick_rows_batch_p0(int F, uint32_t C, uint32_t W, bool wave8);
+RowsBatchFn pick_rows_batch_p1(int F, uint32_t C, uint32_t W, bool wave8);
+RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W, bool wave8);
+RowsBatchFn pick_rows_batch_p3(int F, uint32_t C, uint32_t W, bool wave8);
 
 // Mask counts served by each instance unit.
 constexpr int kPartFirstF[5] = {0, 4, 7, 10, 13};

@@ -24,20 +24,20 @@ GridBatchFn pick_grid_batch_p0(int F, uint32_t W, uint32_t G, uint32_t C) {
   }
 }
 
-RowsFn pick_rows_p0(int F, uint32_t C, uint32_t W) {
+RowsFn pick_rows_p0(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 1: return rows_kernel_for<1>(C, W);
-    case 2: return rows_kernel_for<2>(C, W);
-    case 3: return rows_kernel_for<3>(C, W);
+    case 1: return rows_kernel_for<1>(C, W, wave8);
+    case 2: return rows_kernel_for<2>(C, W, wave8);
+    case 3: return rows_kernel_for<3>(C, W, wave8);
     default: return nullptr;
   }
 }
 
-RowsBatchFn pick_rows_batch_p0(int F, uint32_t C, uint32_t W) {
+RowsBatchFn pick_rows_batch_p0(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 1: return rows_batch_for<1>(C, W);
-    case 2: return rows_batch_for<2>(C, W);
-    case 3: return rows_batch_for<3>(C, W);
+    case 1: return rows_batch_for<1>(C, W, wave8);
+    case 2: return rows_batch_for<2>(C, W, wave8);
+    case 3: return rows_batch_for<3>(C, W, wave8);
     default: return nullptr;
   }
 }

@@ -22,20 +22,20 @@ GridBatchFn pick_grid_batch_p1(int F, uint32_t W, uint32_t G, uint32_t C) {
   }
 }
 
-RowsFn pick_rows_p1(int F, uint32_t C, uint32_t W) {
+RowsFn pick_rows_p1(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 4: return rows_kernel_for<4>(C, W);
-    case 5: return rows_kernel_for<5>(C, W);
-    case 6: return rows_kernel_for<6>(C, W);
+    case 4: return rows_kernel_for<4>(C, W, wave8);
+    case 5: return rows_kernel_for<5>(C, W, wave8);
+    case 6: return rows_kernel_for<6>(C, W, wave8);
     default: return nullptr;
   }
 }
 
-RowsBatchFn pick_rows_batch_p1(int F, uint32_t C, uint32_t W) {
+RowsBatchFn pick_rows_batch_p1(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 4: return rows_batch_for<4>(C, W);
-    case 5: return rows_batch_for<5>(C, W);
-    case 6: return rows_batch_for<6>(C, W);
+    case 4: return rows_batch_for<4>(C, W, wave8);
+    case 5: return rows_batch_for<5>(C, W, wave8);
+    case 6: return rows_batch_for<6>(C, W, wave8);
     default: return nullptr;
   }
 }

@@ -22,20 +22,20 @@ GridBatchFn pick_grid_batch_p2(int F, uint32_t W, uint32_t G, uint32_t C) {
   }
 }
 
-RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W) {
+RowsFn pick_rows_p2(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 7: return rows_kernel_for<7>(C, W);
-    case 8: return rows_kernel_for<8>(C, W);
-    case 9: return rows_kernel_for<9>(C, W);
+    case 7: return rows_kernel_for<7>(C, W, wave8);
+    case 8: return rows_kernel_for<8>(C, W, wave8);
+    case 9: return rows_kernel_for<9>(C, W, wave8);
     default: return nullptr;
   }
 }
 
-RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W) {
+RowsBatchFn pick_rows_batch_p2(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 7: return rows_batch_for<7>(C, W);
-    case 8: return rows_batch_for<8>(C, W);
-    case 9: return rows_batch_for<9>(C, W);
+    case 7: return rows_batch_for<7>(C, W, wave8);
+    case 8: return rows_batch_for<8>(C, W, wave8);
+    case 9: return rows_batch_for<9>(C, W, wave8);
     default: return nullptr;
   }
 }

@@ -22,18 +22,18 @@ GridBatchFn pick_grid_batch_p3(int F, uint32_t W, uint32_t G, uint32_t C) {
   }
 }
 
-RowsFn pick_rows_p3(int F, uint32_t C, uint32_t W) {
+RowsFn pick_rows_p3(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 10: return rows_kernel_for<10>(C, W);
-    case 11: return rows_kernel_for<11>(C, W);
+    case 10: return rows_kernel_for<10>(C, W, wave8);
+    case 11: return rows_kernel_for<11>(C, W, wave8);
     default: return nullptr;
   }
 }
 
-RowsBatchFn pick_rows_batch_p3(int F, uint32_t C, uint32_t W) {
+RowsBatchFn pick_rows_batch_p3(int F, uint32_t C, uint32_t W, bool wave8) {
   switch (F) {
-    case 10: return rows_batch_for<10>(C, W);
-    case 11: return rows_batch_for<11>(C, W);
+    case 10: return rows_batch_for<10>(C, W, wave8);
+    case 11: return rows_batch_for<11>(C, W, wave8);
     default: return nullptr;
   }
 }

@@ -605,18 +605,23 @@ class GpuSieve:
         return opts
 
     @staticmethod
-    def _choose_layout(P: Plan, strips_log2: int = 19, rounds: int = 3) -> tuple[int, object, str]:
+    def _choose_layout(P: Plan, window_log2: int = 36, rounds: int = 3,
+                       max_s: float = 0.05) -> tuple[int, object, str]:
         """(mi_inner_radix, mi_ring_swap, mi_layout) for the engine: with one layout option,
-        that one; otherwise each option is timed on a window of block 1 and the fastest kept.
+        that one; otherwise each option is timed on windows of block 1 and the fastest kept.
         Every option scans the same positions with the same exact test; only the LDS layout
-        (words per row, masks that fit, strip-lane or row-lane lanes) and so the speed differ.
-        An option the library refuses (a row-lane layout without the strip or steps it needs)
-        is skipped.  Each option's window grows to 2^strips_log2 strips of its own layout
-        (about 8 workgroups per CU, so long strips are not timed on an under-filled GPU),
-        between 2^28 and 2^34 positions; the options within 15 % of the fastest are then timed
-        again on their windows in `rounds` round-robin passes and the best rate of each counts,
-        so that neither the order of the options (the first ones run on a colder GPU) nor one
-        launch's noise decides: single launches vary by several per cent."""
+        (words per row, masks that fit, strip-lane or row-lane lanes, the registers per wave)
+        and so the speed differ.  An option the library refuses (a row-lane layout without the
+        strip or steps it needs) is skipped.  Each option first scans a 2^24-position window
+        untimed (its first launch loads the code and, for an instance that spills, sets up
+        scratch memory, which can take longer than the scan); its timed window then grows from
+        2^24 positions towards min(R / 2, 2^window_log2), each step to what its latest rate
+        scans in max_s (timed_rate), and the options near the best (85 %) are timed again on
+        their windows in `rounds` round-robin passes, each keeping its best rate.  Windows of
+        2^36 positions keep row-lane grids (one wave per unit of a lane group and a superstrip)
+        many waves deep, as in a block scan: smaller ones leave a partial last round of waves
+        that penalises the layouts with the largest units (the k=399 confirmation wheel's lead
+        block ties strip-lane at 2^33 positions and is 3 % faster at 2^36)."""
         opts = GpuSieve._layout_options(P)
         if len(opts) == 1:
             return opts[0]
@@ -634,9 +639,10 @@ class GpuSieve:
                 return pos.value / max(ms.value * 1e-3, 1e-12)
             return scan
 
+        warm = max(1, min(P.R // 2, 1 << 24))
         timed = []          # [option, handle, window, best rate]
         try:
-            best_rate = 0.0
+            best = 0.0
             for bound, swap, lay in opts:
                 try:
                     d = _Desc(P, dict(P.tun, mi_inner_radix=bound, mi_ring_swap=swap,
@@ -647,15 +653,16 @@ class GpuSieve:
                 if L.es_engine_create(C.byref(d.desc), C.byref(h)) != nat.ES_OK:
                     continue
                 timed.append([(bound, swap, lay), h, None, 0.0])
-                info = nat.EngineInfo()
-                nat.check(L.es_engine_info_get(h, C.byref(info)))
-                rate, win = timed_rate(scanner(h), P.R, info.strip_len * info.r0, max_s=0.05,
-                                       floor=0.5 * best_rate, with_window=True)
+                scan = scanner(h)
+                scan((P.R - warm) // 2, (P.R - warm) // 2 + warm)
+                rate, win = timed_rate(scan, P.R, 1, max_s=max_s, strips_log2=window_log2,
+                                       lo_log2=window_log2, hi_log2=window_log2,
+                                       floor=0.5 * best, with_window=True)
                 timed[-1][2:] = [win, rate]
-                best_rate = max(best_rate, rate)
+                best = max(best, rate)
             if not timed:
                 return opts[0]
-            close = [t for t in timed if t[3] >= 0.85 * best_rate]
+            close = [t for t in timed if t[3] >= 0.85 * best]
             for _ in range(rounds if len(close) > 1 else 0):
                 for t in close:
                     t[3] = max(t[3], scanner(t[1])(*t[2]))

@@ -287,6 +287,10 @@ struct es_engine {
   RowPlan* d_rp = nullptr;   // device copy of rp (the row-lane kernel takes a pointer)
   RowsFn rows_kern = nullptr;
   bool wave8 = false;        // row-lane kernel instance sized for 8 waves per SIMD (else 6)
+  // with wave8: the 6-wave instance (no spills, so no scratch memory) and its workgroup size
+  // and LDS bytes, for the scan that allocates nothing
+  RowsFn rows_kern6 = nullptr;
+  uint32_t threads6 = 0, lds_bytes6 = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   uint64_t launches = 0, positions = 0;
@@ -341,20 +345,25 @@ void outer_range(const es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t*
 // the row-lane plan to use (the static-scratch scan).  Strip-lane: one lane per strip;
 // row-lane: one wave per unit (lane group, superstrip).
 int launch_kernel(es_engine* e, hipStream_t st, const ColdPlan* cold, const RowPlan* rows_plan,
-                  uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi) {
+                  uint64_t o_lo, uint64_t o_hi, uint64_t idx_lo, uint64_t idx_hi,
+                  bool no_scratch = false) {
   if (e->rows) {
-    // one wave per unit; superstrips [w_lo, w_hi) in launches of at most max_grid workgroups
+    // one wave per unit; superstrips [w_lo, w_hi) in launches of at most max_grid workgroups;
+    // no_scratch: the 6-wave instance where the engine runs the 8-wave one
+    const bool six = no_scratch && e->wave8;
+    const void* fn = six ? (const void*)e->rows_kern6 : (const void*)e->rows_kern;
+    const uint32_t threads = six ? e->threads6 : e->threads;
+    const uint32_t lds = six ? e->lds_bytes6 : e->lds_bytes;
     const uint64_t span = (uint64_t)e->S * e->V;
     const uint64_t w_end = (o_hi + span - 1) / span;
-    const uint32_t wpg = e->threads / 64;
+    const uint32_t wpg = threads / 64;
     const uint64_t per = std::max<uint64_t>(1, (uint64_t)e->max_grid * wpg / e->Gl);
     const RowPlan* rp = rows_plan ? rows_plan : e->d_rp;
     for (uint64_t w_lo = o_lo / span; w_lo < w_end; w_lo += per) {
       uint64_t a = w_lo, b = std::min(w_end, w_lo + per);
       const uint64_t grid = ((uint64_t)e->Gl * (b - a) + wpg - 1) / wpg;
       void* args[] = {&rp, &a, &b, &o_lo, &o_hi, &idx_lo, &idx_hi};
-      ES_HIP(hipLaunchKernel((const void*)e->rows_kern, dim3((uint32_t)grid), dim3(e->threads),
-                             args, e->lds_bytes, st));
+      ES_HIP(hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(threads), args, lds, st));
     }
   } else {
     uint64_t u_lo = o_lo / e->S, u_hi = (o_hi + e->S - 1) / e->S;
@@ -983,7 +992,7 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
     if (rows && W == 1) cands.push_back({(const void*)pick_rows((int)e->F, e->kind, W, true), true});
     const std::vector<uint32_t> sizes = e->threads ? std::vector<uint32_t>{e->threads}
                                                    : std::vector<uint32_t>{256u, 512u, 1024u};
-    uint32_t best_waves = 0, best_t = 0;
+    uint32_t best_waves = 0, best_t = 0, best6_waves = 0, best6_t = 0;
     bool best_w8 = false;
     for (const Cand& c : cands) {
       if (!c.fn) continue;
@@ -995,16 +1004,25 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
                                      (int)prop.sharedMemPerBlock));
         int nb = 0;
         ES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, c.fn, (int)t, lds_t));
-        if ((uint32_t)nb * (t / 64) > best_waves) {
-          best_waves = (uint32_t)nb * (t / 64);
+        const uint32_t waves = (uint32_t)nb * (t / 64);
+        if (waves > best_waves) {
+          best_waves = waves;
           best_t = t;
           best_w8 = c.wave8;
         }
+        if (!c.wave8 && waves > best6_waves) {
+          best6_waves = waves;
+          best6_t = t;
+        }
       }
     }
+    const uint32_t given = e->threads;
     if (!e->threads) e->threads = best_t ? best_t : 256;  // nothing fits: reported below
     if (best_w8) {
       e->wave8 = true;
+      e->rows_kern6 = e->rows_kern;
+      e->threads6 = given ? given : (best6_t ? best6_t : e->threads);
+      e->lds_bytes6 = table_bytes + (e->threads6 / 64) * kWaveQueueBytes;
       e->rows_kern = pick_rows((int)e->F, e->kind, W, true);
     }
   }
@@ -1876,7 +1894,7 @@ extern "C" int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint6
     per = (per + e->lead - 1) / e->lead * e->lead;
     for (uint64_t a = o_lo; a < o_hi; a += per) {
       rc = launch_kernel(e, st, cold_dev, reinterpret_cast<const RowPlan*>(base + kStaticRowOff),
-                         a, std::min(o_hi, a + per), idx_lo, idx_hi);
+                         a, std::min(o_hi, a + per), idx_lo, idx_hi, /*no_scratch=*/true);
       if (rc) return rc;
     }
   }

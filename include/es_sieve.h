@@ -185,7 +185,11 @@ int es_engine_kernel_time(es_engine* e, double* ms, uint64_t* launches, uint64_t
  * scratch (counters, the minimum's slot, one 16-byte residue record per survivor slot, the
  * exact test's plan records) is a file-scope device array of 1 MiB + 1 KiB that every GPU
  * holds until the process ends.  cap above ES_STATIC_MAX_IDS fails with ES_ERR_LIMIT.  Calls
- * that overlap on one GPU race on the shared scratch.  No counterpart in the reference: it is
+ * that overlap on one GPU race on the shared scratch.  Where the engine runs the row-lane kernel
+ * instance sized for 8 waves per SIMD (its spills use per-lane scratch memory, which the HIP
+ * runtime sets up at launch), this scan runs the 6-wave instance, which has none; the rarely
+ * used instances that spill anyway (three- or four-word rows, runtime word counts, 7 or more
+ * masks in some forms) keep their scratch.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (HIP graph capture, tight serving loops). */
 #define ES_STATIC_MAX_IDS 65536u
 int es_engine_scan_window_static(es_engine* e, uint64_t idx_lo, uint64_t idx_hi, uint64_t* ids,

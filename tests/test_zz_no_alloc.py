@@ -67,3 +67,35 @@ def test_static_scan_equals_allocating_scan_twice():
     finally:
         hip.hipFree(d_ids)
         hip.hipFree(d_res)
+
+
+K399_PRIMARY = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1], [137, 1],
+                [41, 1], [2, 9], [37, 1], [29, 1], [101, 1], [13, 1], [31, 1], [67, 1], [5, 2]]
+
+
+def test_static_scan_row_lane_engine():
+    """A row-lane engine that runs the 8-wave kernel instance (1024-thread workgroups): the
+    static scan runs the spill-free 6-wave instance and returns the same survivors and minimum,
+    on a dense filter prefix, twice, the second window smaller."""
+    import gpu_sieve_mi355x as G
+    hip = _hip()
+    plan = EP.Plan(399, {"max_survivors": 1 << 20, "mi_filter_limit": 4}, K399_PRIMARY)
+    eng = G.GpuSieve(plan)
+    assert eng.layout == "rows"
+    cap = 65536
+    d_ids, d_res = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_ids), 8 * cap) == 0
+    assert hip.hipMalloc(C.byref(d_res), 32) == 0
+    try:
+        lo = 123456789
+        for t, a, b in ((6, lo, lo + (1 << 22)), (1, lo + 77, lo + 77 + (1 << 18))):
+            print(f"case static scan k=399 rows threads={eng.threads} t={t} window=[{a},{b})",
+                  flush=True)
+            want_ids, want_min = eng.scan_window(t, a, b)
+            n, got, mn = _static_scan(hip, eng, a, b, d_ids, d_res, cap)
+            assert 0 < n <= cap and n == len(want_ids)
+            assert np.array_equal(got, np.sort(np.asarray(want_ids, dtype=np.uint64)))
+            assert (None if mn is None else t * plan.M + mn) == want_min
+    finally:
+        hip.hipFree(d_ids)
+        hip.hipFree(d_res)

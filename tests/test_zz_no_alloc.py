@@ -74,28 +74,33 @@ K399_PRIMARY = [[107, 1], [3, 6], [211, 1], [139, 1], [83, 1], [59, 1], [103, 1]
 
 
 def test_static_scan_row_lane_engine():
-    """A row-lane engine that runs the 8-wave kernel instance (1024-thread workgroups): the
-    static scan runs the spill-free 6-wave instance and returns the same survivors and minimum,
-    on a dense filter prefix, twice, the second window smaller."""
+    """The k=399 primary wheel's engine (row-lane, the kernel instance sized for 8 waves per
+    SIMD, as the bench runs it): the static scan runs the spill-free 6-wave instance and returns
+    the same survivors and minimum on windows of block 6 around g(399), twice, the second window
+    smaller; the minimum is g(399) itself."""
     import gpu_sieve_mi355x as G
+    from conftest import bfile
     hip = _hip()
-    plan = EP.Plan(399, {"max_survivors": 1 << 20, "mi_filter_limit": 4}, K399_PRIMARY)
+    plan = EP.Plan(399, {"max_survivors": 1 << 20}, K399_PRIMARY)
     eng = G.GpuSieve(plan)
-    assert eng.layout == "rows"
+    assert eng.layout == "rows" and eng.threads == 1024
+    g = bfile()[399]
+    gi, gt = plan.n_to_idx(g)
+    assert gt == 6
     cap = 65536
     d_ids, d_res = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(d_ids), 8 * cap) == 0
     assert hip.hipMalloc(C.byref(d_res), 32) == 0
     try:
-        lo = 123456789
-        for t, a, b in ((6, lo, lo + (1 << 22)), (1, lo + 77, lo + 77 + (1 << 18))):
-            print(f"case static scan k=399 rows threads={eng.threads} t={t} window=[{a},{b})",
+        for half in (1 << 31, 1 << 20):
+            a, b = gi - half, gi + half
+            print(f"case static scan k=399 rows threads={eng.threads} t=6 window=[{a},{b})",
                   flush=True)
-            want_ids, want_min = eng.scan_window(t, a, b)
+            want_ids, want_min = eng.scan_window(6, a, b)
             n, got, mn = _static_scan(hip, eng, a, b, d_ids, d_res, cap)
-            assert 0 < n <= cap and n == len(want_ids)
+            assert n >= 1 and n == len(want_ids)
             assert np.array_equal(got, np.sort(np.asarray(want_ids, dtype=np.uint64)))
-            assert (None if mn is None else t * plan.M + mn) == want_min
+            assert 6 * plan.M + mn == want_min == g
     finally:
         hip.hipFree(d_ids)
         hip.hipFree(d_res)

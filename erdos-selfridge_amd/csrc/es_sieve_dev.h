@@ -616,6 +616,25 @@ __device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* 
                 "wider rows read the byte carry table and have no lead block");
   const uint32_t wb = W > 1 ? P.word_bytes : 0u;
   uint32_t cnt = 0;
+  // Each lane holds its latest candidate word (bits, step and word) in registers; `pend` marks
+  // the lanes holding one (wave-uniform).  Candidate words are rare per lane (about one in 64
+  // steps at k=399), so the wave appends all pending entries to its LDS queue at once, when a
+  // lane finds a second one, instead of one LDS write per step that has any.
+  uint32_t pbits = 0, py = 0;
+  uint64_t pend = 0;
+  auto flush = [&]() {
+    if ((pend >> lane) & 1ull) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(pend >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pend, 0u));
+      wq[cnt + below] = make_uint2(pbits, py);
+    }
+    cnt += (uint32_t)__popcll(pend);
+    pend = 0;
+    if (cnt >= 64u) {  // at most 127 queued: a drain follows every flush that passes 64
+      cnt -= 64u;
+      es_rows_drain<C, W>(P, wq, cnt, 64u, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
+    }
+  };
   for (uint32_t v = 0; v < V; ++v) {
     cu32_t* st = (cu32_t*)(P.steps) + (size_t)v * kStepDwords;
     const uint64_t zl = (uint64_t)st[1] | ((uint64_t)st[2] << 32);
@@ -664,20 +683,16 @@ __device__ __forceinline__ void es_rows_sweep(const RowPlan& P, const uint32_t* 
       // padding lanes (a repeated row) are left out through the wave's mask of valid lanes
       const uint64_t any = __ballot(bits != 0u) & vmask;
       if (any) {
+        if (any & pend) flush();   // a lane's slot is taken: every pending entry to the queue
         if (bits && valid) {
-          const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
-          wq[cnt + below] = make_uint2(
-              bits, v | ((uint32_t)w << kRowsWordShift) | (lane << kQueueIndexBits));
+          pbits = bits;
+          py = v | ((uint32_t)w << kRowsWordShift) | (lane << kQueueIndexBits);
         }
-        cnt += (uint32_t)__popcll(any);
-        if (cnt >= 64u) {  // at most 127 queued: a drain follows every word's append
-          cnt -= 64u;
-          es_rows_drain<C, W>(P, wq, cnt, 64u, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
-        }
+        pend |= any;
       }
     }
   }
+  if (pend) flush();
   if (cnt) es_rows_drain<C, W>(P, wq, 0, cnt, Hp, Lpl, Lph, sg, X, obase, idx_lo, idx_hi);
 }
 
@@ -820,7 +835,7 @@ es_rows_batch_kernel(const RowPlan* __restrict__ plans, const GridJob* __restric
     if (jobs[mid].wg_start <= b) lo = mid; else hi = mid;
   }
   const GridJob J = jobs[lo];
-  const RowPlan P = plans[J.plan];
+  const RowPlan& P = plans[J.plan];
   for (uint32_t z = threadIdx.x; z < P.lds_words; z += blockDim.x) lds[z] = P.lds_src[z];
   __syncthreads();
   const uint32_t wpg = blockDim.x >> 6;

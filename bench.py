@@ -319,8 +319,8 @@ LDS_PEAK_NOMINAL = 256 * 256 * 2.4e9
 # + WRITE_SIZE, the copy calibration): es_rows_kernel<6, 1, 1> (row-lane layout, the default) and
 # es_sieve_kernel<6, 0, 1, 1> (strip-lane).  Other instances are not profiled.
 K399_PROFILE = {
-    "rows": {"sq_lds_idx_active_per_wave_step": 29.49, "sq_lds_bank_conflict_per_wave_step": 12.60,
-             "sq_insts_valu_per_wave_step": 33.1, "hbm_bytes_per_launch": 91.2e6},
+    "rows": {"sq_lds_idx_active_per_wave_step": 28.38, "sq_lds_bank_conflict_per_wave_step": 12.60,
+             "sq_insts_valu_per_wave_step": 32.78, "hbm_bytes_per_launch": 335e6},
     "strips": {"sq_lds_idx_active_per_wave_step": 39.7, "sq_lds_bank_conflict_per_wave_step": 24.0,
                "sq_insts_valu_per_wave_step": 35.5, "hbm_bytes_per_launch": 1.47e6}}
 

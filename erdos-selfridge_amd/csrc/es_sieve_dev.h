@@ -543,8 +543,10 @@ struct RowPlan {
 
 __device__ __forceinline__ uint2* es_rows_queue(const RowPlan& P, const uint32_t* lds) {
   const uint32_t base = (P.lds_words * 4u + 15u) & ~15u;
+  // the wave's index is uniform across its lanes: a scalar register holds the queue's address
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   return reinterpret_cast<uint2*>(reinterpret_cast<char*>(const_cast<uint32_t*>(lds)) + base +
-                                  (threadIdx.x >> 6) * kWaveQueueBytes);
+                                  wave * kWaveQueueBytes);
 }
 
 // h and l of n's residue at step row `st` for a lane with superstrip-plus-strip-row sums

@@ -41,8 +41,10 @@ INT32_PEAK = 256 * 128 * 2.4e9
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    # defaults: 50 timed steps after 20 warm-up steps for the headline (the device's clocks ramp
+    # over its first ~10 launches of 2^36 positions, 2.27 -> 1.93 ms), 20 after 2 elsewhere
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--window-log2", type=int, default=36)
     ap.add_argument("--masks", type=int, default=None)
     ap.add_argument("--strip", type=int, default=None)
@@ -81,6 +83,11 @@ def main() -> int:
     ap.add_argument("--mitm-k", type=int, default=200)
     ap.add_argument("--mitm-mem", type=float, default=64.0, help="mitm: table budget, GB")
     a = ap.parse_args()
+    headline = a.workload == "sieve"
+    if a.steps is None:
+        a.steps = 50 if headline else 20
+    if a.warmup is None:
+        a.warmup = 20 if headline else 2
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(blocks and outer-index ranges shard by construction; see the sharding notes)",

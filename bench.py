@@ -41,8 +41,9 @@ INT32_PEAK = 256 * 128 * 2.4e9
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    # defaults: 50 timed steps after 20 warm-up steps for the headline (the device's clocks ramp
-    # over its first ~10 launches of 2^36 positions, 2.27 -> 1.93 ms), 20 after 2 elsewhere
+    # defaults: 50 timed steps after 20 warm-up steps for the 2^36-position windows of the
+    # headline and k200 (the device's clocks ramp over its first ~10 such launches, 2.33 -> 1.93
+    # ms), 20 after 2 elsewhere
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--window-log2", type=int, default=36)
@@ -83,7 +84,7 @@ def main() -> int:
     ap.add_argument("--mitm-k", type=int, default=200)
     ap.add_argument("--mitm-mem", type=float, default=64.0, help="mitm: table budget, GB")
     a = ap.parse_args()
-    headline = a.workload == "sieve"
+    headline = a.workload in ("sieve", "k200")   # windows of 2^36 positions per step
     if a.steps is None:
         a.steps = 50 if headline else 20
     if a.warmup is None:

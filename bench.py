@@ -271,7 +271,8 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
     E_pos x ns wins.  Both wheels then run the full search (GpuSieve.search: blocks t = 0, 1, ... until
     the referee accepts a survivor) in interleaved pairs (plain, surrogate, surrogate, plain,
     ...; harness/paired_bench.py:22-68), each must return the b-file g(k), and each wheel's
-    time is its median over the pairs.  Engine construction is outside the timed search."""
+    time is its median over the pairs.  Engine construction and one full-size warm-up window
+    per engine are outside the timed searches."""
     import es_data
     import es_plan
     import es_surrogate as S
@@ -289,6 +290,9 @@ def first_block_ab(train_ks=tuple(range(300, 321, 2)), eval_ks=(301, 305, 309, 3
                                probe_log2=33, seed=k)
         wheels = {"plain": res["plain"]["wheel"], "surrogate": res["best"]["wheel"]}
         engines = {tag: G.GpuSieve(es_plan.Plan(k, None, w)) for tag, w in wheels.items()}
+        for eng in engines.values():     # one full-size window each, outside the timed searches
+            eng.scan_window(0, 0, min(1 << 34, eng.plan.R), collect=False)
+            eng.sync()
         times = {"plain": [], "surrogate": []}
         found = {}
         for i in range(pairs):
@@ -803,7 +807,9 @@ def bench_surrogate(a) -> int:
             eng = G.GpuSieve(plan)
             gi, gt = plan.n_to_idx(g)
             lo = max(0, min(plan.R - N, gi - N // 2))
-            eng.scan_window(0, 0, 1 << 26)
+            # warm-up at full size: code load, and scratch memory for every resident wave where
+            # the kernel instance spills, stay outside the timed scans
+            eng.scan_window(0, 0, min(N, plan.R))
             eng.sync()
             eng.kernel_time()
             w0 = time.perf_counter()

@@ -612,9 +612,10 @@ class GpuSieve:
         Every option scans the same positions with the same exact test; only the LDS layout
         (words per row, masks that fit, strip-lane or row-lane lanes, the registers per wave)
         and so the speed differ.  An option the library refuses (a row-lane layout without the
-        strip or steps it needs) is skipped.  Each option first scans a 2^24-position window
-        untimed (its first launch loads the code and, for an instance that spills, sets up
-        scratch memory, which can take longer than the scan); its timed window then grows from
+        strip or steps it needs) is skipped.  Each option first runs its window sequence once
+        untimed (its first launches load the code and, for an instance that spills, have the
+        runtime set up scratch memory for as many waves as the largest window keeps resident,
+        which can take longer than the scans); its timed window then grows from
         2^24 positions towards min(R / 2, 2^window_log2), each step to what its latest rate
         scans in max_s (timed_rate), and the options near the best (85 %) are timed again on
         their windows in `rounds` round-robin passes, each keeping its best rate.  Windows of
@@ -639,7 +640,6 @@ class GpuSieve:
                 return pos.value / max(ms.value * 1e-3, 1e-12)
             return scan
 
-        warm = max(1, min(P.R // 2, 1 << 24))
         timed = []          # [option, handle, window, best rate]
         try:
             best = 0.0
@@ -654,10 +654,10 @@ class GpuSieve:
                     continue
                 timed.append([(bound, swap, lay), h, None, 0.0])
                 scan = scanner(h)
-                scan((P.R - warm) // 2, (P.R - warm) // 2 + warm)
-                rate, win = timed_rate(scan, P.R, 1, max_s=max_s, strips_log2=window_log2,
-                                       lo_log2=window_log2, hi_log2=window_log2,
-                                       floor=0.5 * best, with_window=True)
+                grow = dict(max_s=max_s, strips_log2=window_log2, lo_log2=window_log2,
+                            hi_log2=window_log2)
+                timed_rate(scan, P.R, 1, **grow)            # warm-up pass, untimed
+                rate, win = timed_rate(scan, P.R, 1, floor=0.5 * best, with_window=True, **grow)
                 timed[-1][2:] = [win, rate]
                 best = max(best, rate)
             if not timed:

@@ -90,7 +90,7 @@ def main() -> int:
     if a.warmup is None:
         a.warmup = 20 if headline else 2
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the engine has no multi-GPU path "
               "(blocks and outer-index ranges shard by construction; see the sharding notes)",
               file=sys.stderr)
         return 2

@@ -657,14 +657,17 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // Carry layout.  Slots are sorted by e_i, so the slots of row h with h + e_i >= q0 are a
   // suffix [j(h), r0).  W = 1 and q0 <= 256: the table holds j(h) as one byte per h and the
   // kernel builds the carry word from it: the table spans at most 64 dwords, one per LDS
-  // bank, so its random reads never conflict (7 % faster at k=399, q0 = 107, than word rows;
-  // with q0 > 256 the bytes conflict too and the extra instructions cost 3 %).  Otherwise W <=
+  // bank, so its random reads never conflict (7 % faster at k=399, q0 = 107, than word rows).
+  // With q0 > 256 the bytes conflict too: the strip-lane kernel keeps word rows (the extra
+  // instructions cost it 3 %), the row-lane kernel takes the bytes (a quarter of the banks'
+  // load of word rows: the k=300..320 batch, 11 of whose 21 wheels have q0 = 343, 361 or 512,
+  // ran 3 % faster).  Otherwise W <=
   // 2: one row of W words per h (bit i set iff j(h) <= i < r0).  W > 2: j(h) as u16, each
   // carry word built from it (word rows would not fit).  Row-lane rows of two words: j(h) <=
   // 64 as one byte per h, each word's carry built from it.
   auto carry_form = [&](bool rw) -> uint32_t {
     if (rw && W >= 2) return 1u;
-    return (W == 1 && q0 <= 256 && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
+    return (W == 1 && (q0 <= 256 || rw) && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;
   };
   uint32_t jbytes = carry_form(rows);
   e->kind = (jbytes == 1 ? 1u : 0u) | (e->lead > 1 ? 2u : 0u);

@@ -211,7 +211,8 @@ def choose_masks(plan: Plan, n_masks: int, limit: int | None = None,
             cands.append((rate, fi, d, p ** d))
     cands.sort(key=lambda c: (c[0], c[1], -c[2]))
     # carry table: q0 rows of `words` words, or q0 u16 suffix starts when words > 2; row-lane
-    # rows of several words: q0 bytes
+    # rows of several words: q0 bytes (the one-word row-lane table is q0 bytes too: the q0
+    # words counted for it leave a margin)
     if rows and words > 1:
         carry = (q0 + 3) // 4
     else:
@@ -806,13 +807,13 @@ class GpuSieve:
         """Algorithmic LDS bytes the sieve kernel reads per residue position: per outer index
         (r0 positions of the kernel's ring 0) each lane reads one mask row of 8 bytes per
         ring-0 word from each of the masks staged in LDS, and its carry entry -- one byte of the
-        j(h) table (one-word rows, q0 <= 256, every mask in LDS; every row-lane row of two to
-        four words), a row of one 4-byte word per ring-0 word (two words), or a u16 j(h)
-        (wider rows).  The candidate queue (a few entries per thousand outer indices) and the
+        j(h) table (every row-lane layout; strip-lane one-word rows with q0 <= 256 and every
+        mask in LDS), a row of one 4-byte word per ring-0 word (other rows of one or two
+        words), or a u16 j(h) (wider rows).  The candidate queue (a few entries per thousand outer indices) and the
         per-workgroup table staging are left out."""
         info = self.info
         W, F, G = int(info.words), int(info.n_masks_used), int(info.global_masks)
-        if (W == 1 and info.q0 <= 256 and G == 0) or (W > 1 and info.layout == 2):
+        if info.layout == 2 or (W == 1 and info.q0 <= 256 and G == 0):
             carry = 1
         elif W <= 2:
             carry = 4 * W

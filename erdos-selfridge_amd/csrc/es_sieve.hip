@@ -661,10 +661,10 @@ int es_engine_create(const es_plan_desc* d, es_engine** out) {
   // With q0 > 256 the bytes conflict too: the strip-lane kernel keeps word rows (the extra
   // instructions cost it 3 %), the row-lane kernel takes the bytes (a quarter of the banks'
   // load of word rows: the k=300..320 batch, 11 of whose 21 wheels have q0 = 343, 361 or 512,
-  // ran 3 % faster).  Otherwise W <=
-  // 2: one row of W words per h (bit i set iff j(h) <= i < r0).  W > 2: j(h) as u16, each
-  // carry word built from it (word rows would not fit).  Row-lane rows of two words: j(h) <=
-  // 64 as one byte per h, each word's carry built from it.
+  // ran 3 % faster).  Otherwise W <= 2: one row of W words per h (bit i set iff
+  // j(h) <= i < r0).  W > 2: j(h) as u16, each carry word built from it (word rows would not
+  // fit).  Row-lane rows of two words: j(h) <= 64 as one byte per h, each word's carry built
+  // from it.
   auto carry_form = [&](bool rw) -> uint32_t {
     if (rw && W >= 2) return 1u;
     return (W == 1 && (q0 <= 256 || rw) && d->global_masks == 0) ? 1u : W <= 2 ? 0u : 2u;

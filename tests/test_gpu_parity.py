@@ -323,15 +323,20 @@ def test_auto_workgroup_size(k):
         window_check(k, wheel, 1, lo, lo + span, limit=8, threads=th, **keep)
 
 
-@pytest.mark.parametrize("k,tune", [(399, {}), (399, {"mi_global_masks": 2}), (300, {})])
+@pytest.mark.parametrize("k,tune", [(399, {}), (399, {"mi_global_masks": 2}),
+                                    (300, {"mi_layout": "strips"}), (300, {"mi_layout": "rows"})])
 def test_carry_table_forms(k, tune):
     """Both ring-0 carry forms of one-word rows scan the oracle's survivors: the byte table of
     suffix starts (k=399, q0 = 107 <= 256), word rows (the same wheel with two masks served
-    from global memory, an instance without the byte form) and word rows by size (k=300,
-    q0 = 2^9 > 256)."""
+    from global memory, an instance without the byte form), word rows by size in the
+    strip-lane layout (k=300, q0 = 2^9 > 256) and the byte table at that size in the row-lane
+    layout."""
     wheel = K399_PRIMARY if k == 399 else None
     plan = EP.Plan(k, None, wheel)
-    rng = random.Random(7 * k + len(tune))
+    if "mi_layout" in tune:
+        eng, _ = engine(k, wheel, mi_filter_limit=8, **tune)
+        assert eng.layout == tune["mi_layout"] and eng.info.q0 > 256 and eng.info.words == 1
+    rng = random.Random(7 * k + len(str(tune)))
     span = min(1 << 22, plan.R // 2)
     lo = rng.randrange(0, plan.R - span)
     window_check(k, wheel, 3, lo, lo + span, limit=8, **tune)

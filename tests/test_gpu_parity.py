@@ -143,6 +143,21 @@ def test_bfile_gate(ks):
         assert res["g"] == b[k], (k, res["g"], b[k])
 
 
+@pytest.mark.parametrize("ks", [(201, 240), (241, 280), (281, 320), (321, 360), (361, 400)])
+def test_default_wheels_every_k(ks):
+    """Every default-planner wheel of k = 201..400, as the engine sets it up by itself (its own
+    layout timing, masks and ring order), scans a seeded window of 2^22 positions in a seeded
+    block to the oracle's survivors and minimum.  The first 8 filters leave 15 to several
+    hundred survivors per window (median 124, 28,983 over the 200 windows), so the masks, the
+    carry forms and the exact re-test all carry weight."""
+    for k in range(ks[0], ks[1] + 1):
+        plan = EP.Plan(k, None)
+        rng = random.Random(1000 + k)
+        span = min(1 << 22, plan.R // 2)
+        lo = rng.randrange(0, plan.R - span)
+        window_check(k, None, rng.randrange(0, 1 << 20), lo, lo + span, limit=8)
+
+
 def _frontier():
     return json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
 

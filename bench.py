@@ -52,7 +52,9 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=None)
     ap.add_argument("--layout", choices=("auto", "strips", "rows"), default=None,
                     help="sieve: lane layout of the kernel (speed only)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target CPU time of the oracle's sample (its positions near g(399) "
+                         "run faster than the probe's, so the sample takes ~0.7 of this)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--part2-candidates", type=int, default=10000,
@@ -425,10 +427,15 @@ def cpu_baseline(plan, eng, g_idx: int, a) -> dict:
     hc = host_cpu()
     threads = a.cpu_threads or hc["threads"]
     lo = g_idx - 50_000
-    probe = 1 << 20
-    c0 = time.perf_counter()
-    O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + probe, threads=threads)
-    per = (time.perf_counter() - c0) / probe
+    # seconds per position from two probes, so that the scan's fixed set-up (filter tables,
+    # threads) does not count: the sample then takes about cpu_seconds
+    probe = 1 << 23
+    dt = []
+    for n in (probe, 2 * probe):
+        c0 = time.perf_counter()
+        O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + n, threads=threads)
+        dt.append(time.perf_counter() - c0)
+    per = max(dt[1] - dt[0], dt[1] / 4) / probe
     span = max(probe, min(int(a.cpu_seconds / per), 1 << 36))
     c0 = time.perf_counter()
     n_cpu, ids_cpu = O.c_scan(K, plan.wheel, plan.rest, BLOCK, lo, lo + span, threads=threads)

@@ -158,6 +158,54 @@ def test_default_wheels_every_k(ks):
         window_check(k, None, rng.randrange(0, 1 << 20), lo, lo + span, limit=8)
 
 
+def test_random_tunables():
+    """150 seeded random cases: k in [30, 400], the default wheel under random engine tunables
+    (layout, mask count, workgroup size, strip target, ring order, masks in global memory,
+    filter prefix) and a random window of 1 to 2^22 positions in a random block, against the
+    oracle's survivors and minimum.  The only set-up allowed to fail is a forced row-lane layout
+    that the wheel cannot take, with the engine's own message."""
+    import gpu_sieve_mi355x as G
+    rng = random.Random(20261204)
+    ran = 0
+    for i in range(150):
+        k = rng.randrange(30, 401)
+        tune = {"max_survivors": 1 << 22}
+        if rng.random() < 0.5:
+            tune["mi_layout"] = rng.choice(["auto", "strips", "rows"])
+        if rng.random() < 0.4:
+            tune["mi_masks"] = rng.randrange(1, 9)
+        if rng.random() < 0.3:
+            tune["threads"] = rng.choice([256, 512, 1024])
+        if rng.random() < 0.3:
+            tune["mi_strip_target"] = rng.choice([256, 512, 2048, 4096])
+        if rng.random() < 0.3:
+            tune["mi_ring_swap"] = rng.choice(["auto", 0, 1])
+        if rng.random() < 0.15:
+            tune["mi_global_masks"] = rng.choice([0, 2, 4])
+        limit = rng.choice([None, 3, 5, 8, 12])
+        if limit is not None:
+            tune["mi_filter_limit"] = limit
+        plan = EP.Plan(k, dict(tune))
+        span = min(rng.choice([1, 100, 4096, 1 << 16, 1 << 20, 1 << 22]), plan.R)
+        lo = rng.randrange(0, plan.R - span + 1)
+        t = rng.randrange(0, 1 << 20)
+        say(f"case random {i} k={k} t={t} window=[{lo},{lo + span}) tune={tune}")
+        try:
+            eng = G.GpuSieve(plan)
+        except AssertionError as e:   # the engine's set-up errors, as the reference's asserts
+            assert tune.get("mi_layout") == "rows" and "row-lane layout needs" in str(e), e
+            continue
+        ids, mn = eng.scan_window(t, lo, lo + span)
+        filt = plan.rest if limit is None else plan.rest[:limit]
+        n, want, wmin = O.c_scan_min(k, plan.wheel, filt, t, lo, lo + span, threads=16,
+                                     cap=1 << 22)
+        got = np.sort(np.asarray(ids, dtype=np.uint64))
+        assert len(got) == n and np.array_equal(got, want)
+        assert mn == wmin
+        ran += 1
+    assert ran >= 120
+
+
 def _frontier():
     return json.load(open(os.path.join(GOLDEN, "runs_frontier.json")))["terms"]
 

@@ -140,3 +140,37 @@ def test_config3_full_size_autotuned():
         print(f"  job k={plan.k} t={t} oracle window [{sub}, +2^28)", flush=True)
         n, ids = O.c_scan(plan.k, plan.wheel, plan.rest, t, sub, sub + (1 << 28), threads=16)
         assert [i for i in got if sub <= i < sub + (1 << 28)] == [int(x) for x in ids]
+
+
+def test_batch_random_mixes():
+    """30 seeded random batches of 1..8 engines (k in [30, 400], filter prefixes of 4..12,
+    forced ring orders and layouts) with 1..3 windows each of 1 to 2^22 positions in random
+    blocks: every job returns the oracle's survivors."""
+    import es_batch
+    import es_oracle as O
+    import gpu_sieve_mi355x as G
+    rng = random.Random(5)
+    for r in range(30):
+        b = es_batch.Batch(out_cap_per_job=1 << 20)
+        jobs = []
+        for _ in range(rng.randrange(1, 9)):
+            k = rng.randrange(30, 401)
+            tune = {"max_survivors": 1 << 20, "mi_filter_limit": rng.choice([4, 6, 8, 12])}
+            if rng.random() < 0.3:
+                tune["mi_ring_swap"] = rng.choice([0, 1])
+            if rng.random() < 0.3:
+                tune["mi_layout"] = rng.choice(["strips", "auto"])
+            eng = G.GpuSieve(EP.Plan(k, dict(tune)))
+            for _ in range(rng.randrange(1, 4)):
+                span = min(rng.choice([1, 1000, 1 << 16, 1 << 20, 1 << 22]), eng.plan.R)
+                lo = rng.randrange(0, eng.plan.R - span + 1)
+                t = rng.randrange(1 << 20)
+                jobs.append((b.add(eng, t, lo, lo + span), eng, k, tune, t, lo, lo + span))
+        print(f"case batch random {r}: " + " ".join(f"k={k}/t={t}/[{lo},{hi})"
+                                                     for _, _, k, _, t, lo, hi in jobs),
+              flush=True)
+        b.run()
+        for jid, eng, k, tune, t, lo, hi in jobs:
+            filt = eng.plan.rest[:tune["mi_filter_limit"]]
+            n, want = O.c_scan(k, eng.plan.wheel, filt, t, lo, hi, threads=16, cap=1 << 20)
+            assert b.collect(jid) == [int(x) for x in want], (r, k, t, lo, hi)
